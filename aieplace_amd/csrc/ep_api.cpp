@@ -1,0 +1,991 @@
+// ep_api.cpp -- the C ABI (include/eplace.h): context set-up, per-iteration
+// calls and the Nesterov optimiser driving the device kernels.
+//
+// Reference interfaces replaced: Placer (vck5000/host/src/include/AIEplace.h:21-99,
+// AIEplace.cpp:9-138, 993-1070) and GraphDriver (GraphDriver.h:16-65).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ep_ctx.h"
+
+namespace ep {
+
+char *last_error_buffer()
+{
+    static thread_local char buf[512] = {0};
+    return buf;
+}
+
+namespace {
+
+#define EP_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(EP_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
+                        __FILE__, __LINE__);                                                 \
+    } while (0)
+
+template <class T>
+ep_status dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(EP_ENOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T), hipGetErrorString(e));
+    return EP_OK;
+}
+
+template <class T>
+ep_status upload(T *dst, const T *src, size_t n)
+{
+    if (n == 0) return EP_OK;
+    EP_HIP(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return EP_OK;
+}
+
+uint64_t rng_next(uint64_t &s)
+{
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double rng_uniform(uint64_t &s) { return (double)(rng_next(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+int lanes_for_degree(int d)
+{
+    int g = 2;
+    while (g < d && g < 64) g <<= 1;
+    return g;
+}
+
+// Exact overlap of fixed cells with the bins (Bin.h:78-121), host FP64, once.
+void fixed_map(const Ctx &c, const std::vector<float2> &wh, const std::vector<float2> &pos,
+               std::vector<float> &out)
+{
+    std::vector<double> m((size_t)c.Mx * c.My, 0.0);
+    const double bw = c.bw, bh = c.bh;
+    for (int k = c.n_place; k < c.n_all; ++k) {
+        const double x0 = pos[k].x, x1 = x0 + wh[k].x, y0 = pos[k].y, y1 = y0 + wh[k].y;
+        int i0 = std::max(0, (int)std::floor((x0 - c.xl) / bw));
+        int i1 = std::min(c.Mx - 1, (int)std::floor((x1 - c.xl) / bw));
+        int j0 = std::max(0, (int)std::floor((y0 - c.yl) / bh));
+        int j1 = std::min(c.My - 1, (int)std::floor((y1 - c.yl) / bh));
+        for (int i = i0; i <= i1; ++i) {
+            const double ox = std::min(c.xl + (i + 1) * bw, x1) - std::max(c.xl + i * bw, x0);
+            if (ox <= 0) continue;
+            for (int j = j0; j <= j1; ++j) {
+                const double oy = std::min(c.yl + (j + 1) * bh, y1) - std::max(c.yl + j * bh, y0);
+                if (oy <= 0) continue;
+                m[(size_t)i * c.My + j] += ox * oy;
+            }
+        }
+    }
+    out.resize(m.size());
+    for (size_t b = 0; b < m.size(); ++b) out[b] = (float)m[b];
+}
+
+double gamma_of(const Ctx &c, double tau)
+{
+    double g = (double)c.prm.gamma_bins * c.bw;
+    if (c.prm.gamma_schedule) g *= std::pow(10.0, (tau - 0.1) * 20.0 / 9.0 - 1.0);
+    return g;
+}
+
+ep_status check_launch()
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(EP_EDEVICE, "kernel launch failed: %s", hipGetErrorString(e));
+    return EP_OK;
+}
+
+// Stage the host scalars; the copy is part of the stream (and of a captured graph).
+void push_scalars(Ctx &c, hipStream_t s)
+{
+    (void)hipMemcpyAsync(c.d_scal, c.h_scal, sizeof(DevScalars), hipMemcpyHostToDevice, s);
+}
+
+// One gradient evaluation at `pos`: density + overflow, WA, field, gather.
+void enqueue_eval(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref, float2 *gout,
+                  float2 *gw_out, float2 *f_out, hipStream_t s)
+{
+    launch_fill_maps(c, s);
+    launch_density(c, pos, s);
+    launch_overflow(c, s);
+    launch_wa(c, pos, s);
+    launch_field(c, c.prm.precision, s);
+    launch_cell_grad(c, pos, vref, gref, gout, gw_out, f_out, s);
+    launch_finalize(c, s);
+    (void)hipMemcpyAsync(c.h_res, c.d_res, sizeof(DevResult), hipMemcpyDeviceToHost, s);
+}
+
+ep_status sync(Ctx &c)
+{
+    EP_HIP(hipStreamSynchronize(c.stream));
+    return EP_OK;
+}
+
+// Trial step from V[cur]/U[cur]/G[cur] into the other buffers, then evaluate.
+ep_status trial_eval(Ctx &c)
+{
+    const int a = c.cur, b = c.cur ^ 1;
+    if (c.prm.use_graph) {
+        if (!c.graph_built[a]) {
+            hipGraph_t g = nullptr;
+            EP_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            push_scalars(c, c.stream);
+            launch_trial(c, c.V[a], c.U[a], c.G[a], c.V[b], c.U[b], c.stream);
+            enqueue_eval(c, c.V[b], c.V[a], c.G[a], c.G[b], nullptr, nullptr, c.stream);
+            EP_HIP(hipStreamEndCapture(c.stream, &g));
+            hipError_t e = hipGraphInstantiate(&c.graph_exec[a], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(EP_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+            c.graph_built[a] = true;
+        }
+        EP_HIP(hipGraphLaunch(c.graph_exec[a], c.stream));
+    } else {
+        push_scalars(c, c.stream);
+        launch_trial(c, c.V[a], c.U[a], c.G[a], c.V[b], c.U[b], c.stream);
+        enqueue_eval(c, c.V[b], c.V[a], c.G[a], c.G[b], nullptr, nullptr, c.stream);
+        ep_status st = check_launch();
+        if (st != EP_OK) return st;
+    }
+    return sync(c);
+}
+
+}  // namespace
+}  // namespace ep
+
+using namespace ep;
+
+struct ep_ctx : public ep::Ctx {};
+
+extern "C" const char *ep_last_error(void) { return last_error_buffer(); }
+
+extern "C" void ep_default_params(ep_params *p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->bins_x = p->bins_y = 512;
+    p->target_density = 0.9f;
+    p->gamma_bins = 4.0f;
+    p->gamma_schedule = 1;
+    p->lambda_eta = 8e-5f;
+    p->hpwl_ref = 0.0;  // 0: derived from the netlist at ep_create
+    p->mu_max = 1.05f;
+    p->mu_min = 0.95f;
+    p->max_backtrack = 10;
+    p->stop_overflow = 0.10f;
+    p->max_iters = 3000;
+    p->init_disp_bins = 0.1f;
+    p->use_fillers = 1;
+    p->filler_seed = 7;
+    p->precision = EP_PREC_FP32;
+    p->use_graph = 0;
+    p->learned_step = 0;
+}
+
+static ep_status validate(const ep_netlist *nl, const ep_params *p)
+{
+    if (!nl || !p) return fail(EP_EINVAL, "ep_create: null argument");
+    if (nl->n_mov < 1 || nl->n_fix < 0) return fail(EP_EINVAL, "ep_create: need n_mov >= 1, n_fix >= 0");
+    if (!nl->w || !nl->h || !nl->x || !nl->y) return fail(EP_EINVAL, "ep_create: null cell arrays");
+    if (nl->n_nets < 0 || nl->n_pins < 0) return fail(EP_EINVAL, "ep_create: negative net/pin count");
+    if (nl->n_nets > 0 && (!nl->net_start || !nl->pin_cell || !nl->pin_dx || !nl->pin_dy))
+        return fail(EP_EINVAL, "ep_create: null net arrays");
+    if (!(nl->xh > nl->xl) || !(nl->yh > nl->yl)) return fail(EP_EINVAL, "ep_create: empty die");
+    if (p->bins_x < 16 || p->bins_y < 16 || p->bins_x % 16 || p->bins_y % 16)
+        return fail(EP_EINVAL, "ep_create: bins must be multiples of 16 and >= 16 (got %d x %d)",
+                    p->bins_x, p->bins_y);
+    if ((int64_t)p->bins_x * p->bins_y > (int64_t)16384 * 16384)
+        return fail(EP_EINVAL, "ep_create: bin grid too large");
+    if (!(p->target_density > 0.f) || p->target_density > 1.f)
+        return fail(EP_EINVAL, "ep_create: target_density must be in (0, 1]");
+    if (!(p->gamma_bins > 0.f)) return fail(EP_EINVAL, "ep_create: gamma_bins must be > 0");
+    if (p->max_backtrack < 1) return fail(EP_EINVAL, "ep_create: max_backtrack must be >= 1");
+    if (p->precision < EP_PREC_FP32 || p->precision > EP_PREC_FP16X3)
+        return fail(EP_EINVAL, "ep_create: unknown precision %d", p->precision);
+    const int n_all = nl->n_mov + nl->n_fix;
+    for (int k = 0; k < n_all; ++k) {
+        if (!(nl->w[k] > 0.f) || !(nl->h[k] > 0.f) || !std::isfinite(nl->w[k]) || !std::isfinite(nl->h[k]))
+            return fail(EP_EINVAL, "ep_create: cell %d has a non-positive size", k);
+        if (!std::isfinite(nl->x[k]) || !std::isfinite(nl->y[k]))
+            return fail(EP_ENAN, "ep_create: cell %d has a non-finite position", k);
+    }
+    if (nl->n_nets > 0) {
+        if (nl->net_start[0] != 0 || nl->net_start[nl->n_nets] != nl->n_pins)
+            return fail(EP_EINVAL, "ep_create: net_start must run from 0 to n_pins");
+        for (int n = 0; n < nl->n_nets; ++n)
+            if (nl->net_start[n + 1] < nl->net_start[n])
+                return fail(EP_EINVAL, "ep_create: net_start not monotone at net %d", n);
+        for (int p2 = 0; p2 < nl->n_pins; ++p2) {
+            if (nl->pin_cell[p2] < 0 || nl->pin_cell[p2] >= n_all)
+                return fail(EP_EINVAL, "ep_create: pin %d references cell %d out of range", p2, nl->pin_cell[p2]);
+            if (!std::isfinite(nl->pin_dx[p2]) || !std::isfinite(nl->pin_dy[p2]))
+                return fail(EP_ENAN, "ep_create: pin %d has a non-finite offset", p2);
+        }
+    }
+    return EP_OK;
+}
+
+static void destroy_ctx(ep_ctx *c)
+{
+    if (!c) return;
+    for (int i = 0; i < 2; ++i)
+        if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
+    void *ptrs[] = {c->wh, c->net_start, c->pin_cell, c->pin_off, c->pin_grad, c->cell_pin_start,
+                    c->cell_pins, c->V[0], c->V[1], c->U[0], c->U[1], c->G[0], c->G[1],
+                    c->map_field, c->map_mov, c->map_fix, c->ex, c->ey, c->tA, c->tB, c->part,
+                    c->d_scal, c->d_res, c->split_mats, c->split_scratch, c->d_mlp_w};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    field_free(*c);
+    if (c->h_scal) (void)hipHostFree(c->h_scal);
+    if (c->h_res) (void)hipHostFree(c->h_res);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" ep_status ep_plan_fillers(const ep_netlist *nl, const ep_params *p, int32_t *n_fill,
+                                     float *fw_out, float *fh_out, float *x, float *y, int32_t cap)
+{
+    ep_status st = validate(nl, p);
+    if (st != EP_OK) return st;
+    double mov_area = 0, fix_area = 0, wsum = 0, hsum = 0;
+    for (int k = 0; k < nl->n_mov; ++k) {
+        mov_area += (double)nl->w[k] * nl->h[k];
+        wsum += nl->w[k];
+        hsum += nl->h[k];
+    }
+    for (int k = nl->n_mov; k < nl->n_mov + nl->n_fix; ++k) {
+        const double x0 = std::max((double)nl->x[k], (double)nl->xl);
+        const double x1 = std::min((double)nl->x[k] + nl->w[k], (double)nl->xh);
+        const double y0 = std::max((double)nl->y[k], (double)nl->yl);
+        const double y1 = std::min((double)nl->y[k] + nl->h[k], (double)nl->yh);
+        if (x1 > x0 && y1 > y0) fix_area += (x1 - x0) * (y1 - y0);
+    }
+    const double die_area = ((double)nl->xh - nl->xl) * ((double)nl->yh - nl->yl);
+    const float fw = (float)(wsum / nl->n_mov), fh = (float)(hsum / nl->n_mov);
+    int64_t nf = 0;
+    if (p->use_fillers) {
+        const double fa = (double)p->target_density * (die_area - fix_area) - mov_area;
+        if (fa > 0) nf = (int64_t)std::floor(fa / ((double)fw * fh));
+    }
+    if (nf > ((int64_t)1 << 28)) return fail(EP_EINVAL, "ep_plan_fillers: filler count too large");
+    if (n_fill) *n_fill = (int32_t)nf;
+    if (fw_out) *fw_out = fw;
+    if (fh_out) *fh_out = fh;
+    if (x && y) {
+        uint64_t rs = p->filler_seed;
+        const int64_t n = std::min<int64_t>(nf, cap);
+        for (int64_t k = 0; k < n; ++k) {
+            x[k] = (float)(nl->xl + rng_uniform(rs) * ((double)nl->xh - nl->xl - fw));
+            y[k] = (float)(nl->yl + rng_uniform(rs) * ((double)nl->yh - nl->yl - fh));
+        }
+    }
+    return EP_OK;
+}
+
+extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx **out)
+{
+    if (!out) return fail(EP_EINVAL, "ep_create: null output");
+    *out = nullptr;
+    ep_status st = validate(nl, p);
+    if (st != EP_OK) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(EP_EDEVICE, "ep_create: no HIP device available");
+
+    ep_ctx *c = new (std::nothrow) ep_ctx();
+    if (!c) return fail(EP_ENOMEM, "ep_create: host allocation failed");
+    c->prm = *p;
+    (void)hipGetDevice(&c->device);
+    c->Mx = p->bins_x;
+    c->My = p->bins_y;
+    c->xl = nl->xl; c->yl = nl->yl; c->xh = nl->xh; c->yh = nl->yh;
+    c->bw = (float)(((double)nl->xh - nl->xl) / c->Mx);
+    c->bh = (float)(((double)nl->yh - nl->yl) / c->My);
+    c->sqrt2_bw = (float)(M_SQRT2 * (double)c->bw);
+    c->sqrt2_bh = (float)(M_SQRT2 * (double)c->bh);
+    c->n_mov = nl->n_mov;
+    c->n_fix = nl->n_fix;
+
+    // ---- fillers (ePlace whitespace fillers)
+    int32_t n_fill = 0;
+    float fw = 0, fh = 0;
+    if ((st = ep_plan_fillers(nl, p, &n_fill, &fw, &fh, nullptr, nullptr, 0)) != EP_OK) {
+        destroy_ctx(c);
+        return st;
+    }
+    double mov_area = 0;
+    for (int k = 0; k < nl->n_mov; ++k) mov_area += (double)nl->w[k] * nl->h[k];
+    c->mov_area = mov_area;
+    c->n_fill = n_fill;
+    c->n_place = c->n_mov + c->n_fill;
+    c->n_all = c->n_place + c->n_fix;
+
+    // ---- host cell arrays [mov | fill | fix]
+    std::vector<float2> wh(c->n_all), pos(c->n_all);
+    for (int k = 0; k < c->n_mov; ++k) {
+        wh[k] = make_float2(nl->w[k], nl->h[k]);
+        pos[k] = make_float2(nl->x[k], nl->y[k]);
+    }
+    {
+        std::vector<float> fxs(std::max(1, c->n_fill)), fys(std::max(1, c->n_fill));
+        int32_t nf = 0;
+        ep_plan_fillers(nl, p, &nf, &fw, &fh, fxs.data(), fys.data(), c->n_fill);
+        for (int k = 0; k < c->n_fill; ++k) {
+            wh[c->n_mov + k] = make_float2(fw, fh);
+            pos[c->n_mov + k] = make_float2(fxs[k], fys[k]);
+        }
+    }
+    for (int k = 0; k < c->n_fix; ++k) {
+        wh[c->n_place + k] = make_float2(nl->w[nl->n_mov + k], nl->h[nl->n_mov + k]);
+        pos[c->n_place + k] = make_float2(nl->x[nl->n_mov + k], nl->y[nl->n_mov + k]);
+    }
+    // movable initial positions clamped into the die
+    for (int k = 0; k < c->n_place; ++k) {
+        pos[k].x = std::min(std::max(pos[k].x, c->xl), c->xh - wh[k].x);
+        pos[k].y = std::min(std::max(pos[k].y, c->yl), c->yh - wh[k].y);
+    }
+
+    // ---- nets regrouped by lane class (degree < 2 nets go last, untouched)
+    const int N = nl->n_nets;
+    c->n_nets = N;
+    c->n_pins = nl->n_pins;
+    const int lane_classes[kMaxBuckets] = {2, 4, 8, 16, 32, 64};
+    std::vector<std::vector<int32_t>> cls(kMaxBuckets + 1);
+    for (int n = 0; n < N; ++n) {
+        int d = nl->net_start[n + 1] - nl->net_start[n];
+        if (d < 2) {
+            cls[kMaxBuckets].push_back(n);
+            continue;
+        }
+        int g = lanes_for_degree(d), b = 0;
+        while (lane_classes[b] != g) ++b;
+        cls[b].push_back(n);
+    }
+    std::vector<int32_t> net_start(N + 1), pin_cell(nl->n_pins);
+    std::vector<float2> pin_off(nl->n_pins);
+    c->net_perm.clear();
+    c->net_perm.reserve(N);
+    int pin_w = 0, net_w = 0, blk = 0;
+    c->buckets.n = 0;
+    for (int b = 0; b <= kMaxBuckets; ++b) {
+        if (b < kMaxBuckets && !cls[b].empty()) {
+            Bucket &bk = c->buckets.b[c->buckets.n++];
+            bk.lanes = lane_classes[b];
+            bk.net0 = net_w;
+            bk.net1 = net_w + (int32_t)cls[b].size();
+            bk.blk0 = blk;
+            blk += (int)(((int64_t)cls[b].size() * bk.lanes + kBlock - 1) / kBlock);
+        }
+        for (int32_t n : cls[b]) {
+            net_start[net_w] = pin_w;
+            c->net_perm.push_back(n);
+            for (int q = nl->net_start[n]; q < nl->net_start[n + 1]; ++q) {
+                int32_t cell = nl->pin_cell[q];
+                if (cell >= c->n_mov) cell += c->n_fill;  // fixed cells follow the fillers
+                pin_cell[pin_w] = cell;
+                pin_off[pin_w] = make_float2(nl->pin_dx[q], nl->pin_dy[q]);
+                ++pin_w;
+            }
+            ++net_w;
+        }
+    }
+    net_start[N] = pin_w;
+    c->wa_blocks = blk;
+    c->wa_part_blocks = blk;
+
+    // ---- cell -> pin CSR (movable cells)
+    std::vector<int32_t> cps(c->n_mov + 1, 0);
+    for (int q = 0; q < nl->n_pins; ++q)
+        if (pin_cell[q] < c->n_mov) cps[pin_cell[q] + 1]++;
+    for (int k = 0; k < c->n_mov; ++k) cps[k + 1] += cps[k];
+    c->n_pins_mov = cps[c->n_mov];
+    std::vector<int32_t> cpins(std::max(1, c->n_pins_mov)), fillp(cps.begin(), cps.end() - 1);
+    for (int q = 0; q < nl->n_pins; ++q)
+        if (pin_cell[q] < c->n_mov) cpins[fillp[pin_cell[q]]++] = q;
+
+    // ---- hpwl reference of the mu schedule: default scales with the netlist
+    if (!(c->prm.hpwl_ref > 0)) c->prm.hpwl_ref = kHpwlRefPerNetBin * (double)std::max(1, N) * c->bw;
+
+    // ---- device buffers
+    hipError_t he = hipStreamCreate(&c->stream);
+    if (he != hipSuccess) {
+        destroy_ctx(c);
+        return fail(EP_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(he));
+    }
+    const size_t nb = (size_t)c->Mx * c->My;
+    c->ovf_blocks = (int)std::min<size_t>(1024, (nb + kBlock * 4 - 1) / (kBlock * 4));
+    c->cell_blocks = (int)std::min<int64_t>(2048, ((int64_t)c->n_place + kBlock - 1) / kBlock);
+    c->part_wa_off = 0;
+    c->part_ovf_off = 2 * c->wa_part_blocks;
+    c->part_cell_off = c->part_ovf_off + c->ovf_blocks;
+    c->part_total = c->part_cell_off + 7 * c->cell_blocks;
+    std::vector<float> fixmap;
+    fixed_map(*c, wh, pos, fixmap);
+#define EP_TRY(x)                 \
+    do {                          \
+        ep_status s_ = (x);       \
+        if (s_ != EP_OK) {        \
+            destroy_ctx(c);       \
+            return s_;            \
+        }                         \
+    } while (0)
+    EP_TRY(dalloc(&c->wh, c->n_all));
+    EP_TRY(dalloc(&c->net_start, N + 1));
+    EP_TRY(dalloc(&c->pin_cell, nl->n_pins));
+    EP_TRY(dalloc(&c->pin_off, nl->n_pins));
+    EP_TRY(dalloc(&c->pin_grad, nl->n_pins));
+    EP_TRY(dalloc(&c->cell_pin_start, c->n_mov + 1));
+    EP_TRY(dalloc(&c->cell_pins, cpins.size()));
+    for (int i = 0; i < 2; ++i) {
+        EP_TRY(dalloc(&c->V[i], c->n_all));
+        EP_TRY(dalloc(&c->U[i], c->n_place));
+        EP_TRY(dalloc(&c->G[i], c->n_place));
+    }
+    EP_TRY(dalloc(&c->map_field, nb));
+    EP_TRY(dalloc(&c->map_mov, nb));
+    EP_TRY(dalloc(&c->map_fix, nb));
+    EP_TRY(dalloc(&c->ex, nb));
+    EP_TRY(dalloc(&c->ey, nb));
+    EP_TRY(dalloc(&c->tA, 2 * nb));
+    EP_TRY(dalloc(&c->tB, 2 * nb));
+    EP_TRY(dalloc(&c->part, c->part_total));
+    EP_TRY(dalloc(&c->d_scal, 1));
+    EP_TRY(dalloc(&c->d_res, 1));
+    if (hipHostMalloc((void **)&c->h_scal, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_res, sizeof(DevResult), hipHostMallocDefault) != hipSuccess) {
+        destroy_ctx(c);
+        return fail(EP_ENOMEM, "ep_create: pinned host allocation failed");
+    }
+    std::memset(c->h_scal, 0, sizeof(DevScalars));
+    std::memset(c->h_res, 0, sizeof(DevResult));
+    EP_TRY(upload(c->wh, wh.data(), wh.size()));
+    EP_TRY(upload(c->net_start, net_start.data(), net_start.size()));
+    EP_TRY(upload(c->pin_cell, pin_cell.data(), pin_cell.size()));
+    EP_TRY(upload(c->pin_off, pin_off.data(), pin_off.size()));
+    EP_TRY(upload(c->cell_pin_start, cps.data(), cps.size()));
+    EP_TRY(upload(c->cell_pins, cpins.data(), cpins.size()));
+    EP_TRY(upload(c->map_fix, fixmap.data(), fixmap.size()));
+    for (int i = 0; i < 2; ++i) {
+        EP_TRY(upload(c->V[i], pos.data(), pos.size()));
+        EP_TRY(upload(c->U[i], pos.data(), (size_t)c->n_place));
+        if (hipMemset(c->G[i], 0, sizeof(float2) * std::max(1, c->n_place)) != hipSuccess) {
+            destroy_ctx(c);
+            return fail(EP_EDEVICE, "ep_create: hipMemset failed");
+        }
+    }
+    if (hipMemset(c->pin_grad, 0, sizeof(float2) * std::max(1, nl->n_pins)) != hipSuccess ||
+        hipMemset(c->ex, 0, sizeof(float) * nb) != hipSuccess ||
+        hipMemset(c->ey, 0, sizeof(float) * nb) != hipSuccess ||
+        hipMemset(c->part, 0, sizeof(double) * c->part_total) != hipSuccess) {
+        destroy_ctx(c);
+        return fail(EP_EDEVICE, "ep_create: hipMemset failed");
+    }
+    EP_TRY(field_init(*c));
+#undef EP_TRY
+    if (hipDeviceSynchronize() != hipSuccess) {
+        destroy_ctx(c);
+        return fail(EP_EDEVICE, "ep_create: device synchronisation failed");
+    }
+    *out = c;
+    return EP_OK;
+}
+
+extern "C" ep_status ep_destroy(ep_ctx *ctx)
+{
+    if (!ctx) return fail(EP_EINVAL, "ep_destroy: null context");
+    (void)hipStreamSynchronize(ctx->stream);
+    destroy_ctx(ctx);
+    return EP_OK;
+}
+
+extern "C" ep_status ep_get_sizes(ep_ctx *c, int32_t *n_mov, int32_t *n_fill, int32_t *n_fix,
+                                  int32_t *n_nets, int32_t *n_pins)
+{
+    if (!c) return fail(EP_EINVAL, "ep_get_sizes: null context");
+    if (n_mov) *n_mov = c->n_mov;
+    if (n_fill) *n_fill = c->n_fill;
+    if (n_fix) *n_fix = c->n_fix;
+    if (n_nets) *n_nets = c->n_nets;
+    if (n_pins) *n_pins = c->n_pins;
+    return EP_OK;
+}
+
+extern "C" ep_status ep_get_cells(ep_ctx *c, float *w, float *h, float *fix_x, float *fix_y)
+{
+    if (!c) return fail(EP_EINVAL, "ep_get_cells: null context");
+    std::vector<float2> wh(c->n_all), pos(std::max(1, c->n_fix));
+    EP_HIP(hipMemcpy(wh.data(), c->wh, sizeof(float2) * c->n_all, hipMemcpyDeviceToHost));
+    if (c->n_fix)
+        EP_HIP(hipMemcpy(pos.data(), c->V[c->cur] + c->n_place, sizeof(float2) * c->n_fix, hipMemcpyDeviceToHost));
+    for (int k = 0; k < c->n_all; ++k) {
+        if (w) w[k] = wh[k].x;
+        if (h) h[k] = wh[k].y;
+    }
+    for (int k = 0; k < c->n_fix; ++k) {
+        if (fix_x) fix_x[k] = pos[k].x;
+        if (fix_y) fix_y[k] = pos[k].y;
+    }
+    return EP_OK;
+}
+
+extern "C" ep_status ep_get_nets(ep_ctx *c, int32_t *net_start, int32_t *pin_cell, float *pin_dx,
+                                 float *pin_dy, int32_t *net_perm)
+{
+    if (!c) return fail(EP_EINVAL, "ep_get_nets: null context");
+    if (net_start) EP_HIP(hipMemcpy(net_start, c->net_start, sizeof(int32_t) * (c->n_nets + 1), hipMemcpyDeviceToHost));
+    if (pin_cell && c->n_pins) EP_HIP(hipMemcpy(pin_cell, c->pin_cell, sizeof(int32_t) * c->n_pins, hipMemcpyDeviceToHost));
+    if ((pin_dx || pin_dy) && c->n_pins) {
+        std::vector<float2> o(c->n_pins);
+        EP_HIP(hipMemcpy(o.data(), c->pin_off, sizeof(float2) * c->n_pins, hipMemcpyDeviceToHost));
+        for (int q = 0; q < c->n_pins; ++q) {
+            if (pin_dx) pin_dx[q] = o[q].x;
+            if (pin_dy) pin_dy[q] = o[q].y;
+        }
+    }
+    if (net_perm) std::memcpy(net_perm, c->net_perm.data(), sizeof(int32_t) * c->n_nets);
+    return EP_OK;
+}
+
+extern "C" ep_status ep_set_positions(ep_ctx *c, const float *x, const float *y)
+{
+    if (!c || !x || !y) return fail(EP_EINVAL, "ep_set_positions: null argument");
+    std::vector<float2> p(std::max(1, c->n_place)), wh(std::max(1, c->n_place));
+    EP_HIP(hipMemcpy(wh.data(), c->wh, sizeof(float2) * c->n_place, hipMemcpyDeviceToHost));
+    for (int k = 0; k < c->n_place; ++k) {
+        if (!std::isfinite(x[k]) || !std::isfinite(y[k]))
+            return fail(EP_ENAN, "ep_set_positions: non-finite position at cell %d", k);
+        p[k] = make_float2(std::min(std::max(x[k], c->xl), c->xh - wh[k].x),
+                           std::min(std::max(y[k], c->yl), c->yh - wh[k].y));
+    }
+    EP_HIP(hipMemcpy(c->V[c->cur], p.data(), sizeof(float2) * c->n_place, hipMemcpyHostToDevice));
+    EP_HIP(hipMemcpy(c->U[c->cur], p.data(), sizeof(float2) * c->n_place, hipMemcpyHostToDevice));
+    c->opt_ready = false;
+    return EP_OK;
+}
+
+extern "C" ep_status ep_get_positions(ep_ctx *c, float *x, float *y)
+{
+    if (!c || !x || !y) return fail(EP_EINVAL, "ep_get_positions: null argument");
+    std::vector<float2> p(std::max(1, c->n_place));
+    EP_HIP(hipMemcpy(p.data(), c->V[c->cur], sizeof(float2) * c->n_place, hipMemcpyDeviceToHost));
+    for (int k = 0; k < c->n_place; ++k) {
+        x[k] = p[k].x;
+        y[k] = p[k].y;
+    }
+    return EP_OK;
+}
+
+static ep_status run_density(ep_ctx *c)
+{
+    launch_fill_maps(*c, c->stream);
+    launch_density(*c, c->V[c->cur], c->stream);
+    launch_overflow(*c, c->stream);
+    launch_finalize(*c, c->stream);
+    (void)hipMemcpyAsync(c->h_res, c->d_res, sizeof(DevResult), hipMemcpyDeviceToHost, c->stream);
+    ep_status st = check_launch();
+    if (st != EP_OK) return st;
+    return sync(*c);
+}
+
+extern "C" ep_status ep_density(ep_ctx *c, double *overflow)
+{
+    if (!c) return fail(EP_EINVAL, "ep_density: null context");
+    ep_status st = run_density(c);
+    if (st != EP_OK) return st;
+    if (overflow) *overflow = c->h_res->tau;
+    return EP_OK;
+}
+
+extern "C" ep_status ep_get_density(ep_ctx *c, float *field_map, float *mov_map, float *fix_map)
+{
+    if (!c) return fail(EP_EINVAL, "ep_get_density: null context");
+    const size_t nb = (size_t)c->Mx * c->My;
+    if (field_map) EP_HIP(hipMemcpy(field_map, c->map_field, nb * sizeof(float), hipMemcpyDeviceToHost));
+    if (mov_map) EP_HIP(hipMemcpy(mov_map, c->map_mov, nb * sizeof(float), hipMemcpyDeviceToHost));
+    if (fix_map) EP_HIP(hipMemcpy(fix_map, c->map_fix, nb * sizeof(float), hipMemcpyDeviceToHost));
+    return EP_OK;
+}
+
+extern "C" ep_status ep_field(ep_ctx *c, int32_t precision)
+{
+    if (!c) return fail(EP_EINVAL, "ep_field: null context");
+    if (precision < EP_PREC_FP32 || precision > EP_PREC_FP16X3)
+        return fail(EP_EINVAL, "ep_field: unknown precision %d", precision);
+    launch_field(*c, precision, c->stream);
+    ep_status st = check_launch();
+    if (st != EP_OK) return st;
+    return sync(*c);
+}
+
+extern "C" ep_status ep_get_field(ep_ctx *c, float *ex, float *ey)
+{
+    if (!c) return fail(EP_EINVAL, "ep_get_field: null context");
+    const size_t nb = (size_t)c->Mx * c->My;
+    if (ex) EP_HIP(hipMemcpy(ex, c->ex, nb * sizeof(float), hipMemcpyDeviceToHost));
+    if (ey) EP_HIP(hipMemcpy(ey, c->ey, nb * sizeof(float), hipMemcpyDeviceToHost));
+    return EP_OK;
+}
+
+static ep_status copy_pairs(const float2 *d, int n, float *ox, float *oy)
+{
+    if (!ox && !oy) return EP_OK;
+    std::vector<float2> h(std::max(1, n));
+    EP_HIP(hipMemcpy(h.data(), d, sizeof(float2) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; ++k) {
+        if (ox) ox[k] = h[k].x;
+        if (oy) oy[k] = h[k].y;
+    }
+    return EP_OK;
+}
+
+extern "C" ep_status ep_wa_grad(ep_ctx *c, float gamma, double *wa, double *hpwl)
+{
+    if (!c) return fail(EP_EINVAL, "ep_wa_grad: null context");
+    if (!(gamma > 0.f)) return fail(EP_EINVAL, "ep_wa_grad: gamma must be > 0");
+    c->h_scal->gamma_inv = 1.0f / gamma;
+    push_scalars(*c, c->stream);
+    launch_wa(*c, c->V[c->cur], c->stream);
+    launch_finalize(*c, c->stream);
+    (void)hipMemcpyAsync(c->h_res, c->d_res, sizeof(DevResult), hipMemcpyDeviceToHost, c->stream);
+    ep_status st = check_launch();
+    if (st != EP_OK) return st;
+    if ((st = sync(*c)) != EP_OK) return st;
+    if (wa) *wa = c->h_res->wa;
+    if (hpwl) *hpwl = c->h_res->hpwl;
+    return EP_OK;
+}
+
+extern "C" ep_status ep_get_wa_grad(ep_ctx *c, float *gx, float *gy)
+{
+    if (!c) return fail(EP_EINVAL, "ep_get_wa_grad: null context");
+    float2 *gw = nullptr;
+    ep_status st = dalloc(&gw, std::max(1, c->n_place));
+    if (st != EP_OK) return st;
+    c->h_scal->lambda = 0.0f;
+    push_scalars(*c, c->stream);
+    float2 *scratch = c->G[c->cur ^ 1];
+    launch_cell_grad(*c, c->V[c->cur], nullptr, nullptr, scratch, gw, nullptr, c->stream);
+    st = check_launch();
+    if (st == EP_OK) st = sync(*c);
+    if (st == EP_OK) st = copy_pairs(gw, c->n_place, gx, gy);
+    (void)hipFree(gw);
+    return st;
+}
+
+extern "C" ep_status ep_gradient(ep_ctx *c, float gamma, float lambda, float *gx, float *gy,
+                                 float *gwx, float *gwy, float *fx, float *fy, double *hpwl,
+                                 double *overflow)
+{
+    if (!c) return fail(EP_EINVAL, "ep_gradient: null context");
+    if (!(gamma > 0.f)) return fail(EP_EINVAL, "ep_gradient: gamma must be > 0");
+    float2 *gw = nullptr, *f = nullptr;
+    ep_status st = dalloc(&gw, std::max(1, c->n_place));
+    if (st == EP_OK) st = dalloc(&f, std::max(1, c->n_place));
+    if (st != EP_OK) {
+        if (gw) (void)hipFree(gw);
+        return st;
+    }
+    c->h_scal->gamma_inv = 1.0f / gamma;
+    c->h_scal->lambda = lambda;
+    push_scalars(*c, c->stream);
+    enqueue_eval(*c, c->V[c->cur], nullptr, nullptr, c->G[c->cur], gw, f, c->stream);
+    st = check_launch();
+    if (st == EP_OK) st = sync(*c);
+    if (st == EP_OK) st = copy_pairs(c->G[c->cur], c->n_place, gx, gy);
+    if (st == EP_OK) st = copy_pairs(gw, c->n_place, gwx, gwy);
+    if (st == EP_OK) st = copy_pairs(f, c->n_place, fx, fy);
+    (void)hipFree(gw);
+    (void)hipFree(f);
+    if (st != EP_OK) return st;
+    if (hpwl) *hpwl = c->h_res->hpwl;
+    if (overflow) *overflow = c->h_res->tau;
+    c->opt_ready = false;  // G[cur] was overwritten with caller-chosen gamma/lambda
+    return EP_OK;
+}
+
+extern "C" ep_status ep_cell_bins(ep_ctx *c, int32_t stretched, int32_t *out)
+{
+    if (!c || !out) return fail(EP_EINVAL, "ep_cell_bins: null argument");
+    int32_t *d = nullptr;
+    ep_status st = dalloc(&d, 4 * (size_t)std::max(1, c->n_place));
+    if (st != EP_OK) return st;
+    launch_cell_bins(*c, c->V[c->cur], stretched ? 1 : 0, d, c->stream);
+    st = check_launch();
+    if (st == EP_OK) st = sync(*c);
+    if (st == EP_OK) {
+        hipError_t e = hipMemcpy(out, d, sizeof(int32_t) * 4 * (size_t)c->n_place, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = fail(EP_EDEVICE, "ep_cell_bins: copy failed");
+    }
+    (void)hipFree(d);
+    return st;
+}
+
+extern "C" ep_status ep_hpwl(ep_ctx *c, double *hpwl)
+{
+    if (!c || !hpwl) return fail(EP_EINVAL, "ep_hpwl: null argument");
+    return ep_wa_grad(c, c->bw, nullptr, hpwl);
+}
+
+extern "C" ep_status ep_overflow(ep_ctx *c, double *overflow)
+{
+    if (!c || !overflow) return fail(EP_EINVAL, "ep_overflow: null argument");
+    return ep_density(c, overflow);
+}
+
+extern "C" ep_status ep_set_precision(ep_ctx *c, int32_t precision)
+{
+    if (!c) return fail(EP_EINVAL, "ep_set_precision: null context");
+    if (precision < EP_PREC_FP32 || precision > EP_PREC_FP16X3)
+        return fail(EP_EINVAL, "ep_set_precision: unknown precision %d", precision);
+    if (precision != c->prm.precision) {
+        c->prm.precision = precision;
+        for (int i = 0; i < 2; ++i) {
+            if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
+            c->graph_exec[i] = nullptr;
+            c->graph_built[i] = false;
+        }
+    }
+    return EP_OK;
+}
+
+// ---------------------------------------------------------------- optimiser
+
+extern "C" ep_status ep_init_optimizer(ep_ctx *c)
+{
+    if (!c) return fail(EP_EINVAL, "ep_init_optimizer: null context");
+    const int a = c->cur, b = c->cur ^ 1;
+    // U = V (the Nesterov major point starts at the initial placement)
+    EP_HIP(hipMemcpyAsync(c->U[a], c->V[a], sizeof(float2) * c->n_place, hipMemcpyDeviceToDevice, c->stream));
+    ep_status st = run_density(c);
+    if (st != EP_OK) return st;
+    c->gamma = gamma_of(*c, c->h_res->tau);
+    // gradient pieces at v_0 with lambda = 0 give |gW|_1 and |F|_1
+    c->h_scal->gamma_inv = (float)(1.0 / c->gamma);
+    c->h_scal->lambda = 0.0f;
+    push_scalars(*c, c->stream);
+    enqueue_eval(*c, c->V[a], nullptr, nullptr, c->G[a], nullptr, nullptr, c->stream);
+    if ((st = check_launch()) != EP_OK || (st = sync(*c)) != EP_OK) return st;
+    c->evals++;
+    const double sw = c->h_res->sum_gw, sf = c->h_res->sum_f;
+    c->lambda = sf > 0 ? (double)c->prm.lambda_eta * sw / sf : (double)c->prm.lambda_eta;
+    c->prev_hpwl = c->h_res->hpwl;
+    c->last_hpwl = c->h_res->hpwl;
+    c->last_tau = c->h_res->tau;
+    c->last_wa = c->h_res->wa;
+    // preconditioned gradient with lambda_0
+    c->h_scal->lambda = (float)c->lambda;
+    push_scalars(*c, c->stream);
+    launch_cell_grad(*c, c->V[a], nullptr, nullptr, c->G[a], nullptr, nullptr, c->stream);
+    launch_finalize(*c, c->stream);
+    (void)hipMemcpyAsync(c->h_res, c->d_res, sizeof(DevResult), hipMemcpyDeviceToHost, c->stream);
+    if ((st = check_launch()) != EP_OK || (st = sync(*c)) != EP_OK) return st;
+    if (c->h_res->nonfinite) return fail(EP_ENAN, "ep_init_optimizer: non-finite gradient at v_0");
+    // alpha_0 from a probe v_-1 = v_0 - eps g_0
+    const double gmax = c->h_res->gmax;
+    const double eps = gmax > 0 ? (double)c->prm.init_disp_bins * c->bw / gmax : 0.0;
+    c->h_scal->probe_eps = (float)eps;
+    push_scalars(*c, c->stream);
+    launch_probe(*c, c->V[a], c->G[a], c->V[b], c->stream);
+    enqueue_eval(*c, c->V[b], c->V[a], c->G[a], c->G[b], nullptr, nullptr, c->stream);
+    if ((st = check_launch()) != EP_OK || (st = sync(*c)) != EP_OK) return st;
+    c->evals++;
+    const double num = std::sqrt(c->h_res->dv2), den = std::sqrt(c->h_res->dg2);
+    c->alpha = den > 0 ? num / den : eps;
+    if (!(c->alpha > 0) || !std::isfinite(c->alpha)) c->alpha = eps > 0 ? eps : 1e-3;
+    c->ak = 1.0;
+    c->iteration = 0;
+    c->opt_ready = true;
+    return EP_OK;
+}
+
+extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
+{
+    if (!c) return fail(EP_EINVAL, "ep_step: null context");
+    ep_status st;
+    if (!c->opt_ready && (st = ep_init_optimizer(c)) != EP_OK) return st;
+    const double ak1 = 0.5 * (1.0 + std::sqrt(4.0 * c->ak * c->ak + 1.0));
+    const double coef = (c->ak - 1.0) / ak1;
+    double alpha = c->alpha, alpha_hat = alpha;
+    int trials = 0;
+    for (int t = 0; t < c->prm.max_backtrack; ++t) {
+        c->h_scal->alpha = (float)alpha;
+        c->h_scal->coef = (float)coef;
+        c->h_scal->gamma_inv = (float)(1.0 / c->gamma);
+        c->h_scal->lambda = (float)c->lambda;
+        if ((st = trial_eval(*c)) != EP_OK) return st;
+        ++trials;
+        c->evals++;
+        if (c->h_res->nonfinite)
+            return fail(EP_ENAN, "ep_step: non-finite gradient at iteration %d", c->iteration);
+        const double nn = std::sqrt(c->h_res->dv2), dd = std::sqrt(c->h_res->dg2);
+        alpha_hat = dd > 0 ? nn / dd : alpha;
+        if (alpha_hat >= 0.95 * alpha || t == c->prm.max_backtrack - 1) break;
+        alpha = alpha_hat;
+    }
+    // accept: the trial buffers become the current ones
+    c->cur ^= 1;
+    c->alpha = alpha_hat;
+    c->ak = ak1;
+    const double hp = c->h_res->hpwl, tau = c->h_res->tau;
+    c->last_hpwl = hp;
+    c->last_tau = tau;
+    c->last_wa = c->h_res->wa;
+    if (info) {
+        info->iteration = c->iteration;
+        info->evals = trials;
+        info->learned_used = 0;
+        info->learned_accepted = 0;
+        info->hpwl = hp;
+        info->overflow = tau;
+        info->lambda = c->lambda;
+        info->gamma = c->gamma;
+        info->alpha = c->alpha;
+        info->wa = c->h_res->wa;
+    }
+    // lambda (mu schedule) and gamma updates for the next iteration
+    const double dh = hp - c->prev_hpwl;
+    double mu = dh < 0 ? c->prm.mu_max : std::pow((double)c->prm.mu_max, 1.0 - dh / c->prm.hpwl_ref);
+    mu = std::min<double>(std::max<double>(mu, c->prm.mu_min), c->prm.mu_max);
+    c->lambda *= mu;
+    c->prev_hpwl = hp;
+    c->gamma = gamma_of(*c, tau);
+    c->iteration++;
+    return EP_OK;
+}
+
+extern "C" ep_status ep_run(ep_ctx *c, ep_run_info *info, ep_step_info *log, int32_t log_cap)
+{
+    if (!c) return fail(EP_EINVAL, "ep_run: null context");
+    auto t0 = std::chrono::steady_clock::now();
+    ep_status st;
+    const int64_t ev0 = c->evals;
+    if (!c->opt_ready && (st = ep_init_optimizer(c)) != EP_OK) return st;
+    int n = 0;
+    while (c->iteration < c->prm.max_iters) {
+        ep_step_info si;
+        if ((st = ep_step(c, &si)) != EP_OK) return st;
+        if (log && n < log_cap) log[n] = si;
+        ++n;
+        if (si.overflow <= c->prm.stop_overflow) break;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (info) {
+        info->iterations = c->iteration;
+        info->evals = c->evals - ev0;
+        info->learned_proposals = c->learned_proposals;
+        info->learned_accepted = c->learned_accepted;
+        info->hpwl = c->last_hpwl;
+        info->overflow = c->last_tau;
+        info->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return EP_OK;
+}
+
+extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, int32_t cap,
+                                int32_t *n)
+{
+    if (!c || !n || reps < 1) return fail(EP_EINVAL, "ep_profile: bad argument");
+    const int a = c->cur, b = c->cur ^ 1;
+    const double nb = (double)c->Mx * c->My, np = c->n_place, P = c->n_pins, N = c->n_nets;
+    const double pm = c->n_pins_mov;
+    if (c->gamma <= 0) c->gamma = gamma_of(*c, 1.0);
+    c->h_scal->gamma_inv = (float)(1.0 / c->gamma);
+    c->h_scal->lambda = (float)c->lambda;
+    c->h_scal->alpha = (float)(c->alpha > 0 ? c->alpha : 0.0);
+    c->h_scal->coef = 0.0f;
+    push_scalars(*c, c->stream);
+    // field input must be a real map: build it once at the current positions
+    launch_fill_maps(*c, c->stream);
+    launch_density(*c, c->V[a], c->stream);
+    hipEvent_t e0, e1;
+    EP_HIP(hipEventCreate(&e0));
+    EP_HIP(hipEventCreate(&e1));
+    struct Item {
+        const char *name;
+        int launches;
+        double bytes, flops;
+    };
+    const double M = c->Mx;
+    const double gemm_flops = 12.0 * M * M * M * (c->prm.precision == EP_PREC_FP32 ? 1.0 : 3.0);
+    Item items[] = {
+        {"density_scatter", 1, 16.0 * np + 8.0 * nb, 0},
+        {"overflow", 1, 8.0 * nb, 0},
+        {"wa_grad", 1, 4.0 * (N + 1) + 12.0 * P + 8.0 * c->n_all + 8.0 * P, 0},
+        {"field_transforms", 4, 4.0 * 6 * nb + 4.0 * 6 * M * M, gemm_flops},
+        {"cell_grad", 1, 36.0 * np + 12.0 * pm + 4.0 * c->n_mov + 8.0 * nb, 0},
+        {"nesterov_trial", 1, 40.0 * np, 0},
+        {"finalize", 1, 8.0 * c->part_total, 0},
+        {"fill_maps", 1, 12.0 * nb, 0},
+    };
+    int k = 0;
+    for (const Item &it : items) {
+        EP_HIP(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; ++r) {
+            const std::string nm = it.name;
+            if (nm == "density_scatter") launch_density(*c, c->V[a], c->stream);
+            else if (nm == "overflow") launch_overflow(*c, c->stream);
+            else if (nm == "wa_grad") launch_wa(*c, c->V[a], c->stream);
+            else if (nm == "field_transforms") launch_field(*c, c->prm.precision, c->stream);
+            else if (nm == "cell_grad") launch_cell_grad(*c, c->V[a], c->V[a], c->G[a], c->G[b], nullptr, nullptr, c->stream);
+            else if (nm == "nesterov_trial") launch_trial(*c, c->V[a], c->U[a], c->G[a], c->V[b], c->U[b], c->stream);
+            else if (nm == "finalize") launch_finalize(*c, c->stream);
+            else launch_fill_maps(*c, c->stream);
+        }
+        EP_HIP(hipEventRecord(e1, c->stream));
+        EP_HIP(hipEventSynchronize(e1));
+        ep_status st = check_launch();
+        if (st != EP_OK) return st;
+        float ms = 0;
+        EP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (out && k < cap) {
+            std::memset(&out[k], 0, sizeof(ep_kernel_stat));
+            std::snprintf(out[k].name, sizeof(out[k].name), "%s", it.name);
+            out[k].launches = it.launches;
+            out[k].avg_us = 1000.0 * ms / reps;
+            out[k].bytes = it.bytes;
+            out[k].flops = it.flops;
+        }
+        ++k;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *n = k;
+    // leave the maps consistent with the current positions
+    launch_fill_maps(*c, c->stream);
+    launch_density(*c, c->V[a], c->stream);
+    return sync(*c);
+}
+
+extern "C" ep_status ep_take_step_records(ep_ctx *c, float *features, float *log_step, int32_t cap,
+                                          int32_t *n)
+{
+    if (!c || !n) return fail(EP_EINVAL, "ep_take_step_records: null argument");
+    (void)features; (void)log_step; (void)cap;
+    *n = 0;
+    return EP_OK;
+}
+
+extern "C" ep_status ep_set_step_model(ep_ctx *c, const float *w, int32_t n)
+{
+    (void)c; (void)w; (void)n;
+    return fail(EP_ENOTSUP, "ep_set_step_model: learned step not built yet");
+}
+
+extern "C" ep_status ep_wa_packet(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out)
+{
+    (void)in; (void)n_in; (void)gamma; (void)out; (void)n_out;
+    return fail(EP_ENOTSUP, "ep_wa_packet: not built yet");
+}

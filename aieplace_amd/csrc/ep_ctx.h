@@ -1,0 +1,147 @@
+// ep_ctx.h -- device-resident placement context and kernel launch interface.
+//
+// Layout in HBM (n_all = n_mov + n_fill + n_fix, n_place = n_mov + n_fill):
+//   wh[n_all]            float2  cell sizes
+//   V[2][n_all]          float2  lower-left positions: V[cur] = v_k, V[cur^1] =
+//                                trial point; fixed cells' tail copied in both
+//   U[2][n_place]        float2  Nesterov major solution u_k / trial u
+//   G[2][n_place]        float2  preconditioned gradient at V[cur] / at trial
+//   net_start[N+1], pin_cell[P], pin_off[P] (float2): nets regrouped by degree
+//                                class so that each class runs with a fixed
+//                                number of lanes per net
+//   pin_grad[P]          float2  per-pin WA partials (summed per cell by CSR)
+//   cell_pin_start[n_mov+1], cell_pins[P_mov]   cell -> pin CSR
+//   map_field, map_mov, map_fix [Mx*My] float  density maps (x-major)
+//   ex, ey [Mx*My]        float  field; spectral / transform scratch
+//   transform matrices (FP32 and split BF16/FP16 copies)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "ep_common.h"
+
+namespace ep {
+
+constexpr int kBlock = 256;
+// Default HPWL reference of the lambda schedule: this many bin widths per net.
+constexpr double kHpwlRefPerNetBin = 0.05;
+constexpr int kMaxBuckets = 6;  // lanes per net: 2,4,8,16,32,64
+
+struct Bucket {
+    int32_t lanes;  // lanes per net
+    int32_t net0, net1;
+    int32_t blk0;   // first block of this bucket in the WA grid
+};
+
+struct BucketTable {
+    Bucket b[kMaxBuckets];
+    int32_t n;
+};
+
+// Scalars the kernels read each launch; staged from pinned host memory so a
+// captured graph picks up new values on every replay.
+struct DevScalars {
+    float alpha;      // step length of the current trial
+    float coef;       // (a_k - 1)/a_{k+1}
+    float gamma_inv;  // 1/gamma
+    float lambda;     // density weight
+    float probe_eps;  // displacement factor of the alpha_0 probe
+    int32_t pad[3];
+};
+
+// Reductions of one gradient evaluation (FP64), written by the finalize kernel.
+struct DevResult {
+    double hpwl, wa, ovf_num, tau;
+    double dv2, dg2;        // |v' - v|^2, |g' - g|^2
+    double sum_gw, sum_f;   // L1 norms of WA gradient and density force
+    double gmax;            // max |g| component
+    double gnorm2;          // |g'|^2
+    int64_t nonfinite;      // count of non-finite gradient components
+    double pad;
+};
+
+struct Ctx;
+
+// ---- kernel launchers (defined in the .hip files) ----
+void launch_fill_maps(Ctx &c, hipStream_t s);
+void launch_density(Ctx &c, const float2 *pos, hipStream_t s);
+void launch_overflow(Ctx &c, hipStream_t s);
+void launch_wa(Ctx &c, const float2 *pos, hipStream_t s);
+void launch_field(Ctx &c, int precision, hipStream_t s);
+void launch_cell_grad(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref,
+                      float2 *gout, float2 *gw_out, float2 *f_out, hipStream_t s);
+void launch_finalize(Ctx &c, hipStream_t s);
+void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, float2 *v_new,
+                  float2 *u_new, hipStream_t s);
+void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s);
+void launch_cell_bins(Ctx &c, const float2 *pos, int stretched, int32_t *out, hipStream_t s);
+ep_status field_init(Ctx &c);   // builds transform matrices
+void field_free(Ctx &c);
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // sizes
+    int32_t n_mov = 0, n_fill = 0, n_fix = 0, n_all = 0, n_place = 0;
+    int32_t n_nets = 0, n_pins = 0, n_pins_mov = 0;
+    int32_t Mx = 0, My = 0;
+    float xl = 0, yl = 0, xh = 0, yh = 0, bw = 0, bh = 0;
+    float sqrt2_bw = 0, sqrt2_bh = 0;
+    double mov_area = 0;
+    ep_params prm{};
+
+    // netlist (device)
+    float2 *wh = nullptr;
+    int32_t *net_start = nullptr, *pin_cell = nullptr;
+    float2 *pin_off = nullptr, *pin_grad = nullptr;
+    int32_t *cell_pin_start = nullptr, *cell_pins = nullptr;
+    BucketTable buckets{};
+    int32_t wa_blocks = 0;
+    std::vector<int32_t> net_perm;  // stored net k -> original net index
+
+    // optimiser state
+    float2 *V[2] = {nullptr, nullptr}, *U[2] = {nullptr, nullptr}, *G[2] = {nullptr, nullptr};
+    int cur = 0;
+
+    // density + field
+    float *map_field = nullptr, *map_mov = nullptr, *map_fix = nullptr;
+    float *ex = nullptr, *ey = nullptr;
+    float *tA = nullptr, *tB = nullptr;         // transform scratch, 2*Mx*My each
+    float *Cx = nullptr, *Sx_T = nullptr, *Cx_T = nullptr;  // Mx*Mx
+    float *Cy = nullptr, *Sy = nullptr, *Cy_T = nullptr;    // My*My
+    void *split_mats = nullptr;                 // split BF16/FP16 operand copies
+    void *split_scratch = nullptr;              // split data operands
+    int32_t field_precision_built = -1;
+
+    // reductions
+    double *part = nullptr;   // partial sums, see part_* offsets
+    int32_t part_wa_off = 0, part_ovf_off = 0, part_cell_off = 0, part_total = 0;
+    int32_t wa_part_blocks = 0, ovf_blocks = 0, cell_blocks = 0;
+    DevScalars *d_scal = nullptr, *h_scal = nullptr;  // device / pinned host
+    DevResult *d_res = nullptr, *h_res = nullptr;
+
+    // optimiser bookkeeping (host)
+    bool opt_ready = false;
+    double alpha = 0, ak = 1, lambda = 0, gamma = 0, prev_hpwl = 0;
+    double last_hpwl = 0, last_tau = 0, last_wa = 0;
+    int32_t iteration = 0;
+    int64_t evals = 0;
+
+    // graphs: one per parity of `cur`
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    bool graph_built[2] = {false, false};
+
+    // learned step (MLP proposal)
+    float *d_mlp_w = nullptr;
+    bool mlp_loaded = false;
+    std::vector<float> mlp_w_host;
+    std::vector<float> rec_feat, rec_target;   // accepted-step records
+    double feat_hist[8] = {0};                  // previous-iteration scalars
+    int32_t learned_proposals = 0, learned_accepted = 0;
+};
+
+}  // namespace ep

@@ -1,0 +1,677 @@
+// k_place.hip -- HBM-bound kernels of one ePlace gradient evaluation:
+//   density scatter (LDS-privatised bin tiles, atomic flush), overflow
+//   reduction, WA wirelength gradient (per-net lane groups with wavefront
+//   shuffles), per-cell gather + preconditioning, Nesterov trial update, and
+//   the FP64 finalize reduction.
+//
+// Reference algorithms (searsm8/AIEplace): WA terms/partials
+// vck5000/aie/src/partials/compute_abc.cpp:23-125, compute_partials.cpp:11-63,
+// host/src/AIEplace.cpp:612-673; overlap Bin.h:78-121, Grid.cpp:38-110;
+// overflow Bin.h:123-126; HPWL Net.cpp:136-155; cell update AIEplace.cpp:993-1046.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "ep_ctx.h"
+
+namespace ep {
+
+namespace {
+
+constexpr int kTile = 4096;     // bins per LDS-privatised tile (per map)
+constexpr int kCellsPerThread = 4;
+
+struct Geom {
+    float xl, yl, xh, yh, bw, bh, sqrt2_bw, sqrt2_bh;
+    int32_t Mx, My;
+};
+
+__device__ inline Geom geom_of(const float *g4, int Mx, int My)
+{
+    Geom g;
+    g.xl = g4[0];
+    g.yl = g4[1];
+    g.xh = g4[2];
+    g.yh = g4[3];
+    g.bw = g4[4];
+    g.bh = g4[5];
+    g.sqrt2_bw = g4[6];
+    g.sqrt2_bh = g4[7];
+    g.Mx = Mx;
+    g.My = My;
+    return g;
+}
+
+struct GeomArgs {
+    float v[8];
+};
+
+// Rectangle of a cell in bin units plus its clamped bin range.  The float
+// operation sequence is fixed (explicit round-to-nearest intrinsics, no
+// contraction) so bin indices match the FP32 host computation bit for bit.
+struct Rect {
+    float tx0, tx1, ty0, ty1, ratio;
+    int i0, i1, j0, j1;
+};
+
+__device__ inline void bin_range(float x0, float x1, float y0, float y1, const GeomArgs &g,
+                                 int Mx, int My, Rect &r)
+{
+    r.tx0 = __fdiv_rn(__fsub_rn(x0, g.v[0]), g.v[4]);
+    r.tx1 = __fdiv_rn(__fsub_rn(x1, g.v[0]), g.v[4]);
+    r.ty0 = __fdiv_rn(__fsub_rn(y0, g.v[1]), g.v[5]);
+    r.ty1 = __fdiv_rn(__fsub_rn(y1, g.v[1]), g.v[5]);
+    r.i0 = max(0, (int)floorf(r.tx0));
+    r.i1 = min(Mx - 1, (int)floorf(r.tx1));
+    r.j0 = max(0, (int)floorf(r.ty0));
+    r.j1 = min(My - 1, (int)floorf(r.ty1));
+}
+
+// ePlace local smoothing: cells narrower than sqrt(2) bins are widened around
+// their centre and their density scaled by area / smoothed area.
+__device__ inline void stretched_rect(float2 p, float2 s, const GeomArgs &g, int Mx, int My,
+                                      Rect &r)
+{
+    float cx = __fadd_rn(p.x, __fmul_rn(0.5f, s.x));
+    float cy = __fadd_rn(p.y, __fmul_rn(0.5f, s.y));
+    float sx = fmaxf(s.x, g.v[6]);
+    float sy = fmaxf(s.y, g.v[7]);
+    float hx = __fmul_rn(0.5f, sx), hy = __fmul_rn(0.5f, sy);
+    bin_range(__fsub_rn(cx, hx), __fadd_rn(cx, hx), __fsub_rn(cy, hy), __fadd_rn(cy, hy), g, Mx,
+              My, r);
+    r.ratio = (s.x * s.y) / (sx * sy);
+}
+
+__device__ inline void exact_rect(float2 p, float2 s, const GeomArgs &g, int Mx, int My, Rect &r)
+{
+    bin_range(p.x, __fadd_rn(p.x, s.x), p.y, __fadd_rn(p.y, s.y), g, Mx, My, r);
+    r.ratio = 1.0f;
+}
+
+__device__ inline float ovl(float t0, float t1, int i)
+{
+    return fminf(t1, (float)(i + 1)) - fmaxf(t0, (float)i);
+}
+
+__device__ inline double wave_sum(double v)
+{
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__device__ inline double wave_max(double v)
+{
+    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// Block-wide sum (and max) of NV doubles; thread 0 writes out[k].
+template <int NV, int NMAX>
+__device__ inline void block_reduce(double (&v)[NV], double *out)
+{
+    __shared__ double red[kBlock / 64][NV];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = (k < NV - NMAX) ? wave_sum(v[k]) : wave_max(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[wv][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double a = red[0][k];
+            for (int w = 1; w < kBlock / 64; ++w)
+                a = (k < NV - NMAX) ? a + red[w][k] : fmax(a, red[w][k]);
+            out[k] = a;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ maps
+
+__global__ void k_fill_maps(float *__restrict__ map_field, float *__restrict__ map_mov,
+                            const float *__restrict__ map_fix, float rho_t, int nb)
+{
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        map_field[b] = rho_t * map_fix[b];
+        map_mov[b] = 0.0f;
+    }
+}
+
+// Density scatter.  Each block takes kBlock*kCellsPerThread consecutive cells;
+// if the union of their bin ranges fits in kTile bins the block accumulates
+// in LDS and flushes non-zero bins with one global atomic each, otherwise it
+// adds straight into the global maps.
+__global__ __launch_bounds__(kBlock) void k_density(const float2 *__restrict__ pos,
+                                                    const float2 *__restrict__ wh, int n_place,
+                                                    int n_mov, GeomArgs g, int Mx, int My,
+                                                    float *__restrict__ map_field,
+                                                    float *__restrict__ map_mov)
+{
+    __shared__ float tile[2][kTile];
+    __shared__ int box[kBlock / 64][4];
+    const int base = blockIdx.x * kBlock * kCellsPerThread;
+    const float area_scale = g.v[4] * g.v[5];
+
+    int bi0 = INT_MAX, bi1 = -1, bj0 = INT_MAX, bj1 = -1;
+#pragma unroll
+    for (int k = 0; k < kCellsPerThread; ++k) {
+        int c = base + k * kBlock + threadIdx.x;
+        if (c < n_place) {
+            float2 p = pos[c], s = wh[c];
+            Rect r;
+            stretched_rect(p, s, g, Mx, My, r);
+            if (r.i0 <= r.i1 && r.j0 <= r.j1) {
+                bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
+                bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
+            }
+            if (c < n_mov) {
+                exact_rect(p, s, g, Mx, My, r);
+                if (r.i0 <= r.i1 && r.j0 <= r.j1) {
+                    bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
+                    bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
+                }
+            }
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        bi0 = min(bi0, __shfl_xor(bi0, m, 64));
+        bi1 = max(bi1, __shfl_xor(bi1, m, 64));
+        bj0 = min(bj0, __shfl_xor(bj0, m, 64));
+        bj1 = max(bj1, __shfl_xor(bj1, m, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        box[threadIdx.x >> 6][0] = bi0;
+        box[threadIdx.x >> 6][1] = bi1;
+        box[threadIdx.x >> 6][2] = bj0;
+        box[threadIdx.x >> 6][3] = bj1;
+    }
+    __syncthreads();
+    bi0 = box[0][0]; bi1 = box[0][1]; bj0 = box[0][2]; bj1 = box[0][3];
+    for (int w = 1; w < kBlock / 64; ++w) {
+        bi0 = min(bi0, box[w][0]); bi1 = max(bi1, box[w][1]);
+        bj0 = min(bj0, box[w][2]); bj1 = max(bj1, box[w][3]);
+    }
+    if (bi1 < bi0) return;  // whole block off-grid (uniform)
+    const int tw = bj1 - bj0 + 1;
+    const int tsz = (bi1 - bi0 + 1) * tw;
+    const bool lds = tsz <= kTile;
+    if (lds) {
+        for (int t = threadIdx.x; t < tsz; t += kBlock) {
+            tile[0][t] = 0.0f;
+            tile[1][t] = 0.0f;
+        }
+        __syncthreads();
+    }
+#pragma unroll 1
+    for (int k = 0; k < kCellsPerThread; ++k) {
+        int c = base + k * kBlock + threadIdx.x;
+        if (c >= n_place) break;
+        float2 p = pos[c], s = wh[c];
+        Rect r;
+        stretched_rect(p, s, g, Mx, My, r);
+        const float sc = r.ratio * area_scale;
+        for (int i = r.i0; i <= r.i1; ++i) {
+            float ox = ovl(r.tx0, r.tx1, i);
+            if (ox <= 0.0f) continue;
+            for (int j = r.j0; j <= r.j1; ++j) {
+                float oy = ovl(r.ty0, r.ty1, j);
+                if (oy <= 0.0f) continue;
+                float v = ox * oy * sc;
+                if (lds)
+                    atomicAdd(&tile[0][(i - bi0) * tw + (j - bj0)], v);
+                else
+                    unsafeAtomicAdd(&map_field[(size_t)i * My + j], v);
+            }
+        }
+        if (c < n_mov) {
+            exact_rect(p, s, g, Mx, My, r);
+            for (int i = r.i0; i <= r.i1; ++i) {
+                float ox = ovl(r.tx0, r.tx1, i);
+                if (ox <= 0.0f) continue;
+                for (int j = r.j0; j <= r.j1; ++j) {
+                    float oy = ovl(r.ty0, r.ty1, j);
+                    if (oy <= 0.0f) continue;
+                    float v = ox * oy * area_scale;
+                    if (lds)
+                        atomicAdd(&tile[1][(i - bi0) * tw + (j - bj0)], v);
+                    else
+                        unsafeAtomicAdd(&map_mov[(size_t)i * My + j], v);
+                }
+            }
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < tsz; t += kBlock) {
+            const int i = bi0 + t / tw, j = bj0 + t % tw;
+            const size_t b = (size_t)i * My + j;
+            float a = tile[0][t], m = tile[1][t];
+            if (a != 0.0f) unsafeAtomicAdd(&map_field[b], a);
+            if (m != 0.0f) unsafeAtomicAdd(&map_mov[b], m);
+        }
+    }
+}
+
+// Overflow numerator per block: sum_b max(0, mov_b + rho_t*fix_b - rho_t*A_b).
+__global__ __launch_bounds__(kBlock) void k_overflow(const float *__restrict__ map_mov,
+                                                     const float *__restrict__ map_fix, float rho_t,
+                                                     float cap, int nb, double *__restrict__ part)
+{
+    double acc[1] = {0.0};
+    for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock) {
+        float v = map_mov[b] + rho_t * map_fix[b] - cap;
+        if (v > 0.0f) acc[0] += (double)v;
+    }
+    block_reduce<1, 0>(acc, part + blockIdx.x);
+}
+
+// ------------------------------------------------------------------ WA gradient
+
+template <int G>
+__device__ inline float grp_max(float v)
+{
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, G));
+    return v;
+}
+template <int G>
+__device__ inline float grp_min(float v)
+{
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m, G));
+    return v;
+}
+template <int G>
+__device__ inline float grp_sum(float v)
+{
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
+    return v;
+}
+
+// One net per group of G lanes.  Shifted WA form (algebraically equal to
+// AIEplace.cpp:660-668): with d+ = x - x_max, a+ = e^{d+/g}, m+ = sum d+ a+ / sum a+,
+//   dWA+/dx_i = a+_i/b+ * (1 + (d+_i - m+)/g),
+// and symmetrically for the minus side, so FP32 never forms x/g - c/(g b).
+template <int G>
+__device__ inline void wa_net(int s, int e, bool active, int lane, float ginv,
+                              const int32_t *__restrict__ pin_cell,
+                              const float2 *__restrict__ pin_off, const float2 *__restrict__ pos,
+                              float2 *__restrict__ pin_grad, double &hp_acc, double &wa_acc)
+{
+    const int d = e - s;
+    if (G < 64 || d <= G) {
+        // one pin per lane, kept in registers
+        const bool valid = active && lane < d;
+        float px = 0.f, py = 0.f;
+        if (valid) {
+            const int p = s + lane;
+            const int c = pin_cell[p];
+            const float2 o = pin_off[p];
+            const float2 q = pos[c];
+            px = q.x + o.x;
+            py = q.y + o.y;
+        }
+        const float xmx = grp_max<G>(valid ? px : -FLT_MAX);
+        const float xmn = grp_min<G>(valid ? px : FLT_MAX);
+        const float ymx = grp_max<G>(valid ? py : -FLT_MAX);
+        const float ymn = grp_min<G>(valid ? py : FLT_MAX);
+        const float dxp = px - xmx, dxm = px - xmn, dyp = py - ymx, dym = py - ymn;
+        const float axp = valid ? __expf(dxp * ginv) : 0.f;
+        const float axm = valid ? __expf(-dxm * ginv) : 0.f;
+        const float ayp = valid ? __expf(dyp * ginv) : 0.f;
+        const float aym = valid ? __expf(-dym * ginv) : 0.f;
+        const float bxp = grp_sum<G>(axp), bxm = grp_sum<G>(axm);
+        const float byp = grp_sum<G>(ayp), bym = grp_sum<G>(aym);
+        const float cxp = grp_sum<G>(dxp * axp), cxm = grp_sum<G>(dxm * axm);
+        const float cyp = grp_sum<G>(dyp * ayp), cym = grp_sum<G>(dym * aym);
+        const float mxp = cxp / bxp, mxm = cxm / bxm, myp = cyp / byp, mym = cym / bym;
+        if (valid) {
+            const float gx = axp / bxp * (1.f + (dxp - mxp) * ginv) -
+                             axm / bxm * (1.f - (dxm - mxm) * ginv);
+            const float gy = ayp / byp * (1.f + (dyp - myp) * ginv) -
+                             aym / bym * (1.f - (dym - mym) * ginv);
+            pin_grad[s + lane] = make_float2(gx, gy);
+        }
+        if (active && lane == 0) {
+            hp_acc += (double)(xmx - xmn) + (double)(ymx - ymn);
+            wa_acc += (double)((xmx - xmn) + mxp - mxm) + (double)((ymx - ymn) + myp - mym);
+        }
+    } else {
+        // nets wider than a wavefront: three strided passes over the pins
+        float xmx = -FLT_MAX, xmn = FLT_MAX, ymx = -FLT_MAX, ymn = FLT_MAX;
+        for (int p = s + lane; p < e; p += G) {
+            const float2 q = pos[pin_cell[p]], o = pin_off[p];
+            const float px = q.x + o.x, py = q.y + o.y;
+            xmx = fmaxf(xmx, px); xmn = fminf(xmn, px);
+            ymx = fmaxf(ymx, py); ymn = fminf(ymn, py);
+        }
+        xmx = grp_max<G>(xmx); xmn = grp_min<G>(xmn);
+        ymx = grp_max<G>(ymx); ymn = grp_min<G>(ymn);
+        float bxp = 0, bxm = 0, byp = 0, bym = 0, cxp = 0, cxm = 0, cyp = 0, cym = 0;
+        for (int p = s + lane; p < e; p += G) {
+            const float2 q = pos[pin_cell[p]], o = pin_off[p];
+            const float dxp = q.x + o.x - xmx, dxm = q.x + o.x - xmn;
+            const float dyp = q.y + o.y - ymx, dym = q.y + o.y - ymn;
+            const float axp = __expf(dxp * ginv), axm = __expf(-dxm * ginv);
+            const float ayp = __expf(dyp * ginv), aym = __expf(-dym * ginv);
+            bxp += axp; bxm += axm; byp += ayp; bym += aym;
+            cxp += dxp * axp; cxm += dxm * axm; cyp += dyp * ayp; cym += dym * aym;
+        }
+        bxp = grp_sum<G>(bxp); bxm = grp_sum<G>(bxm); byp = grp_sum<G>(byp); bym = grp_sum<G>(bym);
+        cxp = grp_sum<G>(cxp); cxm = grp_sum<G>(cxm); cyp = grp_sum<G>(cyp); cym = grp_sum<G>(cym);
+        const float mxp = cxp / bxp, mxm = cxm / bxm, myp = cyp / byp, mym = cym / bym;
+        for (int p = s + lane; p < e; p += G) {
+            const float2 q = pos[pin_cell[p]], o = pin_off[p];
+            const float dxp = q.x + o.x - xmx, dxm = q.x + o.x - xmn;
+            const float dyp = q.y + o.y - ymx, dym = q.y + o.y - ymn;
+            const float axp = __expf(dxp * ginv), axm = __expf(-dxm * ginv);
+            const float ayp = __expf(dyp * ginv), aym = __expf(-dym * ginv);
+            const float gx = axp / bxp * (1.f + (dxp - mxp) * ginv) -
+                             axm / bxm * (1.f - (dxm - mxm) * ginv);
+            const float gy = ayp / byp * (1.f + (dyp - myp) * ginv) -
+                             aym / bym * (1.f - (dym - mym) * ginv);
+            pin_grad[p] = make_float2(gx, gy);
+        }
+        if (lane == 0) {
+            hp_acc += (double)(xmx - xmn) + (double)(ymx - ymn);
+            wa_acc += (double)((xmx - xmn) + mxp - mxm) + (double)((ymx - ymn) + myp - mym);
+        }
+    }
+}
+
+template <int G>
+__device__ inline void wa_bucket(const Bucket &bk, int blk, const int32_t *__restrict__ net_start,
+                                 const int32_t *__restrict__ pin_cell,
+                                 const float2 *__restrict__ pin_off,
+                                 const float2 *__restrict__ pos, float2 *__restrict__ pin_grad,
+                                 float ginv, double &hp, double &wa)
+{
+    const int net = bk.net0 + (blk * kBlock + (int)threadIdx.x) / G;
+    const int lane = threadIdx.x & (G - 1);
+    const bool active = net < bk.net1;
+    int s = 0, e = 0;
+    if (active) {
+        s = net_start[net];
+        e = net_start[net + 1];
+    }
+    wa_net<G>(s, e, active, lane, ginv, pin_cell, pin_off, pos, pin_grad, hp, wa);
+}
+
+__global__ __launch_bounds__(kBlock) void k_wa(BucketTable bt, const int32_t *__restrict__ net_start,
+                                               const int32_t *__restrict__ pin_cell,
+                                               const float2 *__restrict__ pin_off,
+                                               const float2 *__restrict__ pos,
+                                               float2 *__restrict__ pin_grad,
+                                               const DevScalars *__restrict__ sc,
+                                               double *__restrict__ part)
+{
+    int b = 0;
+    while (b + 1 < bt.n && (int)blockIdx.x >= bt.b[b + 1].blk0) ++b;
+    const Bucket bk = bt.b[b];
+    const int blk = blockIdx.x - bk.blk0;
+    const float ginv = sc->gamma_inv;
+    double hp = 0.0, wa = 0.0;
+    switch (bk.lanes) {
+    case 2: wa_bucket<2>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+    case 4: wa_bucket<4>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+    case 8: wa_bucket<8>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+    case 16: wa_bucket<16>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+    case 32: wa_bucket<32>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+    default: wa_bucket<64>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+    }
+    double v[2] = {hp, wa};
+    block_reduce<2, 0>(v, part + 2 * blockIdx.x);
+}
+
+// ------------------------------------------------------------------ per-cell gather
+
+// g = (gW - lambda * F) / max(1, pins + lambda * area); F = r * sum ov * E over
+// the smoothed rectangle.  Also the FP64 partial norms for the step length.
+__global__ __launch_bounds__(kBlock) void k_cell_grad(
+    const float2 *__restrict__ pos, const float2 *__restrict__ wh, int n_place, int n_mov,
+    GeomArgs g, int Mx, int My, const int32_t *__restrict__ cell_pin_start,
+    const int32_t *__restrict__ cell_pins, const float2 *__restrict__ pin_grad,
+    const float *__restrict__ ex, const float *__restrict__ ey, const float2 *__restrict__ vref,
+    const float2 *__restrict__ gref, float2 *__restrict__ gout, float2 *__restrict__ gw_out,
+    float2 *__restrict__ f_out, const DevScalars *__restrict__ sc, double *__restrict__ part)
+{
+    const float lambda = sc->lambda;
+    const float area_scale = g.v[4] * g.v[5];
+    // sums: dv2, dg2, |gw|, |f|, g2, nonfinite ; max: |g|
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < n_place; c += gridDim.x * kBlock) {
+        const float2 p = pos[c], s = wh[c];
+        float gwx = 0.f, gwy = 0.f;
+        int np = 0;
+        if (c < n_mov) {
+            const int k0 = cell_pin_start[c], k1 = cell_pin_start[c + 1];
+            np = k1 - k0;
+            for (int k = k0; k < k1; ++k) {
+                const float2 pg = pin_grad[cell_pins[k]];
+                gwx += pg.x;
+                gwy += pg.y;
+            }
+        }
+        Rect r;
+        stretched_rect(p, s, g, Mx, My, r);
+        float fx = 0.f, fy = 0.f;
+        for (int i = r.i0; i <= r.i1; ++i) {
+            const float ox = ovl(r.tx0, r.tx1, i);
+            if (ox <= 0.0f) continue;
+            for (int j = r.j0; j <= r.j1; ++j) {
+                const float oy = ovl(r.ty0, r.ty1, j);
+                if (oy <= 0.0f) continue;
+                const size_t b = (size_t)i * My + j;
+                const float o = ox * oy;
+                fx += o * ex[b];
+                fy += o * ey[b];
+            }
+        }
+        const float fs = r.ratio * area_scale;
+        fx *= fs;
+        fy *= fs;
+        const float pre = fmaxf(1.0f, (float)np + lambda * s.x * s.y);
+        const float gx = (gwx - lambda * fx) / pre, gy = (gwy - lambda * fy) / pre;
+        gout[c] = make_float2(gx, gy);
+        if (gw_out) gw_out[c] = make_float2(gwx, gwy);
+        if (f_out) f_out[c] = make_float2(fx, fy);
+        if (vref) {
+            const float2 v0 = vref[c];
+            const double dx = (double)p.x - v0.x, dy = (double)p.y - v0.y;
+            acc[0] += dx * dx + dy * dy;
+        }
+        if (gref) {
+            const float2 g0 = gref[c];
+            const double dx = (double)gx - g0.x, dy = (double)gy - g0.y;
+            acc[1] += dx * dx + dy * dy;
+        }
+        acc[2] += fabs((double)gwx) + fabs((double)gwy);
+        acc[3] += fabs((double)fx) + fabs((double)fy);
+        acc[4] += (double)gx * gx + (double)gy * gy;
+        if (!isfinite(gx) || !isfinite(gy)) acc[5] += 1.0;
+        acc[6] = fmax(acc[6], fmax(fabs((double)gx), fabs((double)gy)));
+    }
+    block_reduce<7, 1>(acc, part + 7 * blockIdx.x);
+}
+
+// Deterministic final reduction of all partial sums (fixed order).
+__global__ __launch_bounds__(kBlock) void k_finalize(const double *__restrict__ part, int wa_off,
+                                                     int wa_blocks, int ovf_off, int ovf_blocks,
+                                                     int cell_off, int cell_blocks, double mov_area,
+                                                     DevResult *__restrict__ res)
+{
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < wa_blocks; b += kBlock) {
+        v[0] += part[wa_off + 2 * b];
+        v[1] += part[wa_off + 2 * b + 1];
+    }
+    for (int b = threadIdx.x; b < ovf_blocks; b += kBlock) v[2] += part[ovf_off + b];
+    for (int b = threadIdx.x; b < cell_blocks; b += kBlock) {
+        const double *q = part + cell_off + 7 * b;
+        v[3] += q[0]; v[4] += q[1]; v[5] += q[2]; v[6] += q[3]; v[7] += q[4]; v[8] += q[5];
+        v[9] = fmax(v[9], q[6]);
+    }
+    __shared__ double out[10];
+    block_reduce<10, 1>(v, out);
+    if (threadIdx.x == 0) {
+        res->hpwl = out[0];
+        res->wa = out[1];
+        res->ovf_num = out[2];
+        res->tau = mov_area > 0 ? out[2] / mov_area : 0.0;
+        res->dv2 = out[3];
+        res->dg2 = out[4];
+        res->sum_gw = out[5];
+        res->sum_f = out[6];
+        res->gnorm2 = out[7];
+        res->nonfinite = (int64_t)out[8];
+        res->gmax = out[9];
+    }
+}
+
+// ------------------------------------------------------------------ Nesterov
+
+__device__ inline float2 clamp_cell(float2 p, float2 s, const GeomArgs &g)
+{
+    p.x = fminf(fmaxf(p.x, g.v[0]), g.v[2] - s.x);
+    p.y = fminf(fmaxf(p.y, g.v[1]), g.v[3] - s.y);
+    return p;
+}
+
+// u' = clamp(v - alpha g); v' = clamp(u' + coef (u' - u))   (ePlace Nesterov step)
+__global__ __launch_bounds__(kBlock) void k_trial(const float2 *__restrict__ v,
+                                                  const float2 *__restrict__ u,
+                                                  const float2 *__restrict__ gr,
+                                                  const float2 *__restrict__ wh, int n_place,
+                                                  GeomArgs g, const DevScalars *__restrict__ sc,
+                                                  float2 *__restrict__ v_new,
+                                                  float2 *__restrict__ u_new)
+{
+    const float alpha = sc->alpha, coef = sc->coef;
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < n_place; c += gridDim.x * kBlock) {
+        const float2 s = wh[c], vv = v[c], uu = u[c], gg = gr[c];
+        float2 un = clamp_cell(make_float2(vv.x - alpha * gg.x, vv.y - alpha * gg.y), s, g);
+        float2 vn = clamp_cell(make_float2(un.x + coef * (un.x - uu.x), un.y + coef * (un.y - uu.y)), s, g);
+        u_new[c] = un;
+        v_new[c] = vn;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_probe(const float2 *__restrict__ v,
+                                                  const float2 *__restrict__ gr,
+                                                  const float2 *__restrict__ wh, int n_place,
+                                                  GeomArgs g, const DevScalars *__restrict__ sc,
+                                                  float2 *__restrict__ v_new)
+{
+    const float eps = sc->probe_eps;
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < n_place; c += gridDim.x * kBlock) {
+        const float2 s = wh[c], vv = v[c], gg = gr[c];
+        v_new[c] = clamp_cell(make_float2(vv.x - eps * gg.x, vv.y - eps * gg.y), s, g);
+    }
+}
+
+__global__ void k_cell_bins(const float2 *__restrict__ pos, const float2 *__restrict__ wh,
+                            int n_place, GeomArgs g, int Mx, int My, int stretched,
+                            int32_t *__restrict__ out)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_place; c += gridDim.x * blockDim.x) {
+        Rect r;
+        if (stretched)
+            stretched_rect(pos[c], wh[c], g, Mx, My, r);
+        else
+            exact_rect(pos[c], wh[c], g, Mx, My, r);
+        out[4 * c + 0] = r.i0;
+        out[4 * c + 1] = r.i1;
+        out[4 * c + 2] = r.j0;
+        out[4 * c + 3] = r.j1;
+    }
+}
+
+GeomArgs geom_args(const Ctx &c)
+{
+    GeomArgs g;
+    g.v[0] = c.xl; g.v[1] = c.yl; g.v[2] = c.xh; g.v[3] = c.yh;
+    g.v[4] = c.bw; g.v[5] = c.bh; g.v[6] = c.sqrt2_bw; g.v[7] = c.sqrt2_bh;
+    return g;
+}
+
+int grid_for(int64_t n, int per_block, int cap)
+{
+    int64_t b = (n + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+
+void launch_fill_maps(Ctx &c, hipStream_t s)
+{
+    const int nb = c.Mx * c.My;
+    hipLaunchKernelGGL(k_fill_maps, dim3(grid_for(nb, kBlock * 4, 4096)), dim3(kBlock), 0, s,
+                       c.map_field, c.map_mov, c.map_fix, c.prm.target_density, nb);
+}
+
+void launch_density(Ctx &c, const float2 *pos, hipStream_t s)
+{
+    const int blocks = grid_for(c.n_place, kBlock * kCellsPerThread, INT_MAX);
+    hipLaunchKernelGGL(k_density, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place, c.n_mov,
+                       geom_args(c), c.Mx, c.My, c.map_field, c.map_mov);
+}
+
+void launch_overflow(Ctx &c, hipStream_t s)
+{
+    const int nb = c.Mx * c.My;
+    const float cap = c.prm.target_density * c.bw * c.bh;
+    hipLaunchKernelGGL(k_overflow, dim3(c.ovf_blocks), dim3(kBlock), 0, s, c.map_mov, c.map_fix,
+                       c.prm.target_density, cap, nb, c.part + c.part_ovf_off);
+}
+
+void launch_wa(Ctx &c, const float2 *pos, hipStream_t s)
+{
+    if (c.wa_blocks == 0) return;
+    hipLaunchKernelGGL(k_wa, dim3(c.wa_blocks), dim3(kBlock), 0, s, c.buckets, c.net_start,
+                       c.pin_cell, c.pin_off, pos, c.pin_grad, c.d_scal, c.part + c.part_wa_off);
+}
+
+void launch_cell_grad(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref,
+                      float2 *gout, float2 *gw_out, float2 *f_out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cell_grad, dim3(c.cell_blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
+                       c.n_mov, geom_args(c), c.Mx, c.My, c.cell_pin_start, c.cell_pins,
+                       c.pin_grad, c.ex, c.ey, vref, gref, gout, gw_out, f_out, c.d_scal,
+                       c.part + c.part_cell_off);
+}
+
+void launch_finalize(Ctx &c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, c.part, c.part_wa_off,
+                       c.wa_part_blocks, c.part_ovf_off, c.ovf_blocks, c.part_cell_off,
+                       c.cell_blocks, c.mov_area, c.d_res);
+}
+
+void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, float2 *v_new,
+                  float2 *u_new, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_trial, dim3(grid_for(c.n_place, kBlock, 8192)), dim3(kBlock), 0, s, v, u,
+                       g, c.wh, c.n_place, geom_args(c), c.d_scal, v_new, u_new);
+}
+
+void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_probe, dim3(grid_for(c.n_place, kBlock, 8192)), dim3(kBlock), 0, s, v, g,
+                       c.wh, c.n_place, geom_args(c), c.d_scal, v_new);
+}
+
+void launch_cell_bins(Ctx &c, const float2 *pos, int stretched, int32_t *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cell_bins, dim3(grid_for(c.n_place, kBlock, 8192)), dim3(kBlock), 0, s,
+                       pos, c.wh, c.n_place, geom_args(c), c.Mx, c.My, stretched, out);
+}
+
+}  // namespace ep

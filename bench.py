@@ -1,0 +1,202 @@
+"""Benchmark: seconds to overflow <= 0.10 on the 2M-cell synthetic placement.
+
+A step is one complete global placement of the workload (initial placement ->
+overflow <= 0.10) with the netlist already resident in HBM.  Each form is timed
+separately over --steps runs after --warmup untimed runs:
+  plain           FP32 transforms, Lipschitz step search (the baseline form)
+  mixed_learned   split-BF16 transforms + MLP-proposed step verified by the
+                  backtracking condition (when built)
+`value` is the headline form's seconds (mixed_learned if present, else plain).
+Prints ONE JSON line.  Runs on one GPU; --gpus > 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "sec to overflow<=0.10 @2M cells, mixed+learned vs plain FP32; HPWL gap %"
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_BF16_MFMA_TFLOPS = 2500.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(A, nl, cfg, form, steps, warmup):
+    from aieplace_amd import api
+    kw = dict(precision=form.get("precision", api.EP_PREC_FP32), use_graph=form.get("use_graph", 1),
+              learned_step=form.get("learned_step", 0))
+    p = A.placer_params(cfg, **kw)
+    pl = A.Placer(nl, p)
+    x0, y0 = pl.get_positions()
+    for w in range(warmup):
+        pl.set_positions(x0, y0)
+        r = pl.run()
+        log(f"[{form['name']}] warmup {w}: {r['iterations']} it, {r['evals']} evals, "
+            f"{r['seconds']:.3f}s, hpwl {r['hpwl']:.6e}, tau {r['overflow']:.4f}")
+    runs = []
+    for k in range(steps):
+        pl.set_positions(x0, y0)
+        t0 = time.perf_counter()
+        r = pl.run()
+        r["wall"] = time.perf_counter() - t0
+        runs.append(r)
+        log(f"[{form['name']}] step {k}: {r['iterations']} it, {r['evals']} evals, "
+            f"{r['seconds']:.3f}s (wall {r['wall']:.3f}s), hpwl {r['hpwl']:.6e}, tau {r['overflow']:.4f}")
+    return pl, runs
+
+
+def profile(pl, reps=20):
+    from aieplace_amd import api
+    stats = (api.EpKernelStat * 16)()
+    n = C.c_int32()
+    api.check(api.lib.ep_profile(pl._ctx, reps, stats, 16, C.byref(n)))
+    out = []
+    for s in stats[:n.value]:
+        out.append(dict(name=s.name.decode(), launches=s.launches, avg_us=s.avg_us,
+                        bytes=s.bytes, flops=s.flops))
+    return out
+
+
+def roofline_of(stats, precision_bf16=False):
+    dom = max(stats, key=lambda s: s["avg_us"])
+    sec = dom["avg_us"] * 1e-6
+    if dom["flops"] > 0:
+        peak = PEAK_BF16_MFMA_TFLOPS if precision_bf16 else PEAK_FP32_MFMA_TFLOPS
+        ach = dom["flops"] / sec / 1e12
+        return dom, dict(bound="mfma", achieved=round(ach, 3), peak=peak, unit="TFLOP/s",
+                         frac=round(ach / peak, 4), traffic=None)
+    ach = dom["bytes"] / sec / 1e9
+    return dom, dict(bound="hbm", achieved=round(ach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                     frac=round(ach / PEAK_HBM_GBS, 4), traffic=None)
+
+
+def cpu_baseline(A, nl, cfg, gpu_evals, budget_s):
+    """FP64 oracle on the host cores over a bounded number of iterations of the
+    same workload; per-evaluation time projected to the GPU plain form's count."""
+    from oracle import oracle as O
+    from aieplace_amd.netlist import plan_fillers
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    p = A.placer_params(cfg)
+    n_fill, fw, fh, fx, fy = plan_fillers(nl, p)
+    n_mov = nl.n_mov
+    w = np.concatenate([nl.w[:n_mov], np.full(n_fill, fw, np.float32), nl.w[n_mov:]])
+    h = np.concatenate([nl.h[:n_mov], np.full(n_fill, fh, np.float32), nl.h[n_mov:]])
+    pc = nl.pin_cell.copy()
+    pc[pc >= n_mov] += n_fill
+    d = O.Design(w=w, h=h, n_mov=n_mov, n_fill=n_fill, fix_x=nl.x[n_mov:], fix_y=nl.y[n_mov:],
+                 net_start=nl.net_start, pin_cell=pc, pin_dx=nl.pin_dx, pin_dy=nl.pin_dy,
+                 die=nl.die, bins=(p.bins_x, p.bins_y))
+    x0 = np.concatenate([nl.x[:n_mov], fx]).astype(np.float64)
+    y0 = np.concatenate([nl.y[:n_mov], fy]).astype(np.float64)
+    bw = (nl.die[2] - nl.die[0]) / p.bins_x
+    op = dict(target_density=p.target_density, gamma_bins=p.gamma_bins,
+              gamma_schedule=p.gamma_schedule, lambda_eta=p.lambda_eta,
+              hpwl_ref=0.05 * nl.n_nets * np.float32(bw), mu_max=p.mu_max, mu_min=p.mu_min,
+              max_backtrack=p.max_backtrack, stop_overflow=p.stop_overflow, max_iters=1,
+              init_disp_bins=p.init_disp_bins)
+    t0 = time.perf_counter()
+    _, _, info = d.place(x0, y0, op)
+    t1 = time.perf_counter() - t0
+    per_eval = t1 / max(1, info["evals"])
+    iters = max(1, int(budget_s / per_eval) - 3)
+    iters = min(iters, 20)
+    if t1 < budget_s / 2 and iters > 1:
+        op["max_iters"] = iters
+        t0 = time.perf_counter()
+        _, _, info = d.place(x0, y0, op)
+        t1 = time.perf_counter() - t0
+        per_eval = t1 / max(1, info["evals"])
+    proj = per_eval * gpu_evals
+    return dict(value=round(proj, 2), unit="s", cores=threads, kind="port",
+                sample=(f"FP64 oracle (oracle/eplace_oracle.c, OpenMP) on the same {nl.n_mov}-cell "
+                        f"instance: {op['max_iters']} Nesterov iterations = {info['evals']} gradient "
+                        f"evaluations in {t1:.1f}s ({per_eval:.3f}s/eval), projected to the GPU plain "
+                        f"form's {gpu_evals} evaluations to overflow<=0.10"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="stdcell_2m")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="plain,mixed_learned")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only for this path)", file=sys.stderr)
+        sys.exit(2)
+
+    import aieplace_amd as A
+    from aieplace_amd import api
+    cfg = A.CONFIGS[args.workload]
+    t0 = time.perf_counter()
+    nl = A.generate(seed=1, **cfg["gen"])
+    log(f"generated {args.workload}: {nl.n_mov} movable, {nl.n_fix} fixed, {nl.n_nets} nets, "
+        f"{nl.n_pins} pins in {time.perf_counter() - t0:.1f}s")
+
+    forms_all = {
+        "plain": dict(name="plain", precision=api.EP_PREC_FP32, learned_step=0),
+        "mixed_learned": dict(name="mixed_learned", precision=api.EP_PREC_BF16X3, learned_step=1),
+    }
+    results, placers = {}, {}
+    for name in args.forms.split(","):
+        form = forms_all[name]
+        try:
+            pl, runs = run_form(A, nl, cfg, form, args.steps, args.warmup)
+        except api.EplaceError as e:
+            if e.code == api.EP_ENOTSUP:
+                log(f"[{name}] not available: {e}")
+                continue
+            raise
+        placers[name] = pl
+        sec = [r["seconds"] for r in runs]
+        results[name] = dict(seconds=round(float(np.mean(sec)), 4), runs=[round(s, 4) for s in sec],
+                             iterations=runs[-1]["iterations"], evals=runs[-1]["evals"],
+                             hpwl=runs[-1]["hpwl"], overflow=round(runs[-1]["overflow"], 4))
+
+    head = "mixed_learned" if "mixed_learned" in results else "plain"
+    plain = results["plain"]
+    out = dict(metric=METRIC, value=results[head]["seconds"], unit="s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=round(results[head]["seconds"] * 1000, 2),
+               higher_is_better=False, scaling="strong", vs_baseline=None,
+               dtype="fp32" if head == "plain" else "bf16x3+fp32", data="synthetic",
+               config=dict(workload=args.workload, movable_cells=nl.n_mov, fixed_cells=nl.n_fix,
+                           nets=nl.n_nets, pins=nl.n_pins, bins=cfg["bins"],
+                           target_density=cfg["target_density"], headline_form=head),
+               forms=results)
+    if head != "plain":
+        m = results[head]
+        out["speedup_vs_plain"] = round(plain["seconds"] / m["seconds"], 3)
+        out["hpwl_gap_pct"] = round(100.0 * (m["hpwl"] - plain["hpwl"]) / plain["hpwl"], 3)
+
+    stats = profile(placers["plain"])
+    dom, roof = roofline_of(stats)
+    out["roofline"] = dict(kernel=dom["name"], **roof)
+    out["kernels"] = {s["name"]: dict(avg_us=round(s["avg_us"], 2), launches=s["launches"])
+                      for s in stats}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(A, nl, cfg, plain["evals"], args.cpu_seconds)
+        out["gpu_vs_cpu_plain"] = round(out["cpu_baseline"]["value"] / plain["seconds"], 1)
+    for pl in placers.values():
+        pl.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

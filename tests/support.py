@@ -1,0 +1,37 @@
+"""Shared helpers: oracle designs built from the same netlist the device gets."""
+from __future__ import annotations
+
+import numpy as np
+
+from aieplace_amd import api
+from aieplace_amd.netlist import Netlist, plan_fillers
+
+
+def oracle_design(O, nl: Netlist, params: api.EpParams):
+    """oracle.Design with the fillers ep_create adds; returns (design, x0, y0)."""
+    n_fill, fw, fh, fx, fy = plan_fillers(nl, params)
+    n_mov, n_fix = nl.n_mov, nl.n_fix
+    w = np.concatenate([nl.w[:n_mov], np.full(n_fill, fw, np.float32), nl.w[n_mov:]])
+    h = np.concatenate([nl.h[:n_mov], np.full(n_fill, fh, np.float32), nl.h[n_mov:]])
+    pc = nl.pin_cell.copy()
+    pc[pc >= n_mov] += n_fill
+    d = O.Design(w=w.astype(np.float64), h=h.astype(np.float64), n_mov=n_mov, n_fill=n_fill,
+                 fix_x=nl.x[n_mov:], fix_y=nl.y[n_mov:], net_start=nl.net_start, pin_cell=pc,
+                 pin_dx=nl.pin_dx, pin_dy=nl.pin_dy, die=nl.die,
+                 bins=(params.bins_x, params.bins_y))
+    x0 = np.concatenate([nl.x[:n_mov], fx]).astype(np.float32)
+    y0 = np.concatenate([nl.y[:n_mov], fy]).astype(np.float32)
+    # same clamp into the die as ep_create
+    x0 = np.minimum(np.maximum(x0, np.float32(nl.die[0])), np.float32(nl.die[2]) - w[:n_mov + n_fill])
+    y0 = np.minimum(np.maximum(y0, np.float32(nl.die[1])), np.float32(nl.die[3]) - h[:n_mov + n_fill])
+    return d, x0.astype(np.float32), y0.astype(np.float32)
+
+
+def oracle_params(params: api.EpParams, bw: float, n_nets: int) -> dict:
+    # ep_create's default when hpwl_ref <= 0: 0.05 bin widths per net
+    href = params.hpwl_ref if params.hpwl_ref > 0 else 0.05 * max(1, n_nets) * np.float32(bw)
+    return dict(target_density=params.target_density, gamma_bins=params.gamma_bins,
+                gamma_schedule=params.gamma_schedule, lambda_eta=params.lambda_eta,
+                hpwl_ref=float(href), mu_max=params.mu_max, mu_min=params.mu_min,
+                max_backtrack=params.max_backtrack, stop_overflow=params.stop_overflow,
+                max_iters=params.max_iters, init_disp_bins=params.init_disp_bins)

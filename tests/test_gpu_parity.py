@@ -1,0 +1,210 @@
+"""Parity of the HIP product path (through the C ABI) against the FP64 oracle.
+
+Bars: bin indices bit-exact; density maps / overflow / HPWL to FP32 rounding;
+WA gradient to 1e-4 of its scale; field within 1e-5 relative (max-norm) of the
+FP64 solve in the plain FP32 form; final HPWL within 0.5% of the FP64
+trajectory at overflow <= 0.10.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import aieplace_amd as A
+from aieplace_amd import api
+from support import oracle_design, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(kind, n=20_000, bins=128, seed=1, **kw):
+    if kind == "mesh":
+        nl = A.generate("mesh", seed=seed)
+        p = A.placer_params(A.CONFIGS["mesh_c1"], **kw)
+    else:
+        nl = A.generate("stdcell", seed=seed, n_movable=n)
+        p = A.placer_params(A.CONFIGS["stdcell_200k"], bins_x=bins, bins_y=bins, **kw)
+    return nl, p
+
+
+def spread_positions(d, rng, frac=1.0):
+    """Random positions over (a fraction of) the die for cells [mov|fill]."""
+    xl, yl, xh, yh = d.die
+    n = d.n_place
+    cx, cy = (xl + xh) / 2, (yl + yh) / 2
+    x = cx + (rng.random(n) - 0.5) * frac * (xh - xl) - d.w[:n] / 2
+    y = cy + (rng.random(n) - 0.5) * frac * (yh - yl) - d.h[:n] / 2
+    x = np.clip(x, xl, xh - d.w[:n]).astype(np.float32)
+    y = np.clip(y, yl, yh - d.h[:n]).astype(np.float32)
+    return x, y
+
+
+CASES = [("mesh", 0, 32), ("stdcell", 20_000, 128), ("stdcell", 20_000, 256)]
+
+
+@pytest.mark.parametrize("kind,n,bins", CASES)
+def test_cell_bins_bit_exact(oracle, kind, n, bins):
+    say(f"case cell_bins {kind} n={n} bins={bins}")
+    nl, p = make(kind, n, bins)
+    d, x0, y0 = oracle_design(oracle, nl, p)
+    with A.Placer(nl, p) as pl:
+        rng = np.random.default_rng(11)
+        x, y = spread_positions(d, rng, 0.6)
+        # put some cells exactly on bin boundaries and on the die edges
+        bw = np.float32((nl.die[2] - nl.die[0]) / bins)
+        k = np.arange(0, d.n_place, 7)
+        x[k] = (np.floor(x[k] / bw) * bw).astype(np.float32)
+        x[::13] = np.float32(nl.die[0])
+        pl.set_positions(x, y)
+        xs, ys = pl.get_positions()
+        w, h = d.w[:d.n_place].astype(np.float32), d.h[:d.n_place].astype(np.float32)
+        bh = np.float32((nl.die[3] - nl.die[1]) / bins)
+        for stretched in (True, False):
+            got = pl.cell_bins(stretched)
+            exp = oracle.cell_bins_f32(xs, ys, w, h, nl.die[0], nl.die[1], bw, bh, bins, bins, stretched)
+            assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("kind,n,bins", CASES)
+def test_density_and_overflow(oracle, kind, n, bins):
+    say(f"case density {kind} n={n} bins={bins}")
+    nl, p = make(kind, n, bins)
+    d, _, _ = oracle_design(oracle, nl, p)
+    with A.Placer(nl, p) as pl:
+        for frac in (0.05, 0.5, 1.0):
+            x, y = spread_positions(d, np.random.default_rng(int(frac * 100)), frac)
+            pl.set_positions(x, y)
+            xs, ys = pl.get_positions()
+            fmap, mmap, fix, tau = pl.computeOverlaps()
+            xd, yd = xs.astype(np.float64), ys.astype(np.float64)
+            ofix = d.density_fixed()
+            ofield = d.density(xd, yd, "stretched", 0, d.n_place) + p.target_density * ofix
+            omov = d.density(xd, yd, "exact", 0, d.n_mov)
+            scale = max(1.0, np.abs(ofield).max())
+            assert np.abs(fmap - ofield).max() <= 2e-5 * scale * max(1, np.sqrt(d.n_place / 1000))
+            assert np.abs(mmap - omov).max() <= 2e-5 * max(1.0, np.abs(omov).max()) * max(1, np.sqrt(d.n_mov / 1000))
+            assert np.abs(fix - ofix).max() <= 1e-5 * max(1.0, ofix.max())
+            otau = d.overflow(omov, ofix, p.target_density)
+            assert tau == pytest.approx(otau, rel=1e-4, abs=1e-6)
+            # area conservation: every smoothed cell lies inside the die when spread
+            if frac <= 0.5:
+                area = (d.w[:d.n_place] * d.h[:d.n_place]).sum() + p.target_density * ofix.sum()
+                assert fmap.sum(dtype=np.float64) == pytest.approx(area, rel=1e-5)
+
+
+@pytest.mark.parametrize("kind,n,bins", CASES)
+def test_wa_gradient_and_hpwl(oracle, kind, n, bins):
+    say(f"case wa {kind} n={n} bins={bins}")
+    nl, p = make(kind, n, bins)
+    d, _, _ = oracle_design(oracle, nl, p)
+    with A.Placer(nl, p) as pl:
+        for frac, gamma in ((0.05, 0.7), (1.0, 3.0), (1.0, 25.0)):
+            x, y = spread_positions(d, np.random.default_rng(3), frac)
+            pl.set_positions(x, y)
+            xs, ys = pl.get_positions()
+            gx, gy, wa, hp = pl.computeAllPartials(gamma)
+            ogx, ogy, owa = d.wa_grad(xs.astype(np.float64), ys.astype(np.float64), gamma)
+            ohp = d.hpwl(xs.astype(np.float64), ys.astype(np.float64))
+            assert hp == pytest.approx(ohp, rel=2e-6)
+            assert wa == pytest.approx(owa, rel=1e-4)
+            sc = max(np.abs(ogx).max(), np.abs(ogy).max())
+            assert np.abs(gx - ogx).max() <= 1e-4 * sc
+            assert np.abs(gy - ogy).max() <= 1e-4 * sc
+            assert abs(gx.sum()) <= 1e-3 * sc * np.sqrt(d.n_mov)  # translation invariance
+
+
+def test_wa_wide_and_degenerate_nets(oracle):
+    """Nets wider than a wavefront (strided path), degree-1 and empty nets."""
+    say("case wa wide/degenerate nets")
+    rng = np.random.default_rng(5)
+    n = 600
+    deg = [2, 1, 0, 100, 65, 64, 33, 3, 200, 2]
+    ns = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    pc = np.concatenate([rng.choice(n, k, replace=False) for k in deg]).astype(np.int32)
+    nl = A.Netlist(w=np.ones(n), h=np.ones(n), x=rng.uniform(0, 60, n), y=rng.uniform(0, 60, n),
+                   n_mov=n, net_start=ns, pin_cell=pc, pin_dx=rng.uniform(0, 1, len(pc)),
+                   pin_dy=rng.uniform(0, 1, len(pc)), die=(0, 0, 64, 64))
+    p = A.placer_params(A.CONFIGS["mesh_c1"], use_fillers=0)
+    d, _, _ = oracle_design(oracle, nl, p)
+    with A.Placer(nl, p) as pl:
+        xs, ys = pl.get_positions()
+        gx, gy, wa, hp = pl.computeAllPartials(2.0)
+        ogx, ogy, owa = d.wa_grad(xs.astype(np.float64), ys.astype(np.float64), 2.0)
+        assert hp == pytest.approx(d.hpwl(xs.astype(np.float64), ys.astype(np.float64)), rel=1e-6)
+        sc = np.abs(ogx).max()
+        assert np.abs(gx - ogx).max() <= 1e-4 * sc and np.abs(gy - ogy).max() <= 1e-4 * sc
+
+
+@pytest.mark.parametrize("bins", [32, 128, 512, 1024])
+def test_field_fp32_vs_fp64(oracle, bins):
+    say(f"case field fp32 bins={bins}")
+    n = {32: 5_000, 128: 20_000, 512: 100_000, 1024: 200_000}[bins]
+    nl, p = make("stdcell", n, bins)
+    d, _, _ = oracle_design(oracle, nl, p)
+    with A.Placer(nl, p) as pl:
+        x, y = spread_positions(d, np.random.default_rng(2), 0.7)
+        pl.set_positions(x, y)
+        fmap, _, _, _ = pl.computeOverlaps()
+        ex, ey = pl.computeElectricFields(api.EP_PREC_FP32)
+        bw = (nl.die[2] - nl.die[0]) / bins
+        rho = fmap.astype(np.float64) / (bw * bw)
+        _, oex, oey = oracle.field(rho, wy_scale=1.0)
+        err = max(np.abs(ex - oex).max() / np.abs(oex).max(), np.abs(ey - oey).max() / np.abs(oey).max())
+        say(f"  field rel err (max-norm) bins={bins}: {err:.3e}")
+        assert err <= 1e-5
+
+
+@pytest.mark.parametrize("kind,n,bins", CASES[:2])
+def test_full_gradient(oracle, kind, n, bins):
+    say(f"case gradient {kind} n={n} bins={bins}")
+    nl, p = make(kind, n, bins)
+    d, _, _ = oracle_design(oracle, nl, p)
+    with A.Placer(nl, p) as pl:
+        x, y = spread_positions(d, np.random.default_rng(8), 0.4)
+        pl.set_positions(x, y)
+        xs, ys = pl.get_positions()
+        bw = (nl.die[2] - nl.die[0]) / bins
+        gamma, lam = 2.0 * bw, 3e-3
+        g = pl.gradient(gamma, lam)
+        o = d.gradient(xs.astype(np.float64), ys.astype(np.float64), p.target_density, gamma, lam)
+        for k in ("gwx", "gwy", "fx", "fy", "gx", "gy"):
+            sc = np.abs(o[k]).max()
+            err = np.abs(g[k] - o[k]).max() / sc
+            say(f"  {k}: rel err {err:.2e}")
+            assert err <= 2e-4, k
+        assert g["hpwl"] == pytest.approx(o["hpwl"], rel=2e-6)
+        assert g["tau"] == pytest.approx(o["tau"], rel=1e-4)
+
+
+@pytest.mark.parametrize("kind,n,bins", [("mesh", 0, 32), ("stdcell", 20_000, 128)])
+def test_placement_trajectory_parity(oracle, kind, n, bins):
+    """Final HPWL at overflow <= 0.10: HIP FP32 vs the FP64 oracle trajectory."""
+    say(f"case placement {kind} n={n} bins={bins}")
+    nl, p = make(kind, n, bins)
+    d, x0, y0 = oracle_design(oracle, nl, p)
+    _, _, oinfo = d.place(x0, y0, oracle_params(p, d.bin_size[0], nl.n_nets))
+    with A.Placer(nl, p) as pl:
+        res = pl.run()
+    say(f"  oracle: it={oinfo['iters']} hpwl={oinfo['hpwl']:.2f} tau={oinfo['overflow']:.4f}; "
+        f"gpu: it={res['iterations']} hpwl={res['hpwl']:.2f} tau={res['overflow']:.4f} {res['seconds']:.3f}s")
+    assert oinfo["overflow"] <= 0.10 and res["overflow"] <= 0.10
+    assert res["hpwl"] == pytest.approx(oinfo["hpwl"], rel=5e-3)
+
+
+def test_graph_replay_matches_eager(oracle):
+    say("case graph replay vs eager")
+    nl, p = make("stdcell", 20_000, 128)
+    recs = {}
+    for g in (0, 1):
+        p.use_graph = g
+        with A.Placer(nl, p) as pl:
+            pl.initializePlacement()
+            recs[g] = [pl.nudgeAllNodes() for _ in range(30)]
+    for a, b in zip(recs[0], recs[1]):
+        assert a["hpwl"] == pytest.approx(b["hpwl"], rel=1e-3)
+        assert a["overflow"] == pytest.approx(b["overflow"], rel=1e-3, abs=1e-4)
