@@ -120,6 +120,7 @@ void enqueue_eval(Ctx &c, const float2 *pos, const float2 *vref, const float2 *g
                   float2 *gw_out, float2 *f_out, hipStream_t s)
 {
     launch_fill_maps(c, s);
+    launch_sort(c, pos, s);
     launch_density(c, pos, s);
     launch_overflow(c, s);
     launch_wa(c, pos, s);
@@ -194,6 +195,14 @@ extern "C" void ep_default_params(ep_params *p)
     p->learned_step = 0;
 }
 
+static ep_status check_precision(int32_t precision, const char *who)
+{
+    if (precision == EP_PREC_FP32 || precision == EP_PREC_BF16X3) return EP_OK;
+    if (precision == EP_PREC_FP16X3)
+        return fail(EP_ENOTSUP, "%s: split-FP16 transforms not available in this build", who);
+    return fail(EP_EINVAL, "%s: unknown precision %d", who, precision);
+}
+
 static ep_status validate(const ep_netlist *nl, const ep_params *p)
 {
     if (!nl || !p) return fail(EP_EINVAL, "ep_create: null argument");
@@ -212,8 +221,9 @@ static ep_status validate(const ep_netlist *nl, const ep_params *p)
         return fail(EP_EINVAL, "ep_create: target_density must be in (0, 1]");
     if (!(p->gamma_bins > 0.f)) return fail(EP_EINVAL, "ep_create: gamma_bins must be > 0");
     if (p->max_backtrack < 1) return fail(EP_EINVAL, "ep_create: max_backtrack must be >= 1");
-    if (p->precision < EP_PREC_FP32 || p->precision > EP_PREC_FP16X3)
-        return fail(EP_EINVAL, "ep_create: unknown precision %d", p->precision);
+    ep_status pst = check_precision(p->precision, "ep_create");
+    if (pst != EP_OK) return pst;
+    if (p->learned_step) return fail(EP_ENOTSUP, "ep_create: learned step not available in this build");
     const int n_all = nl->n_mov + nl->n_fix;
     for (int k = 0; k < n_all; ++k) {
         if (!(nl->w[k] > 0.f) || !(nl->h[k] > 0.f) || !std::isfinite(nl->w[k]) || !std::isfinite(nl->h[k]))
@@ -245,7 +255,8 @@ static void destroy_ctx(ep_ctx *c)
     void *ptrs[] = {c->wh, c->net_start, c->pin_cell, c->pin_off, c->pin_grad, c->cell_pin_start,
                     c->cell_pins, c->V[0], c->V[1], c->U[0], c->U[1], c->G[0], c->G[1],
                     c->map_field, c->map_mov, c->map_fix, c->ex, c->ey, c->tA, c->tB, c->part,
-                    c->d_scal, c->d_res, c->split_mats, c->split_scratch, c->d_mlp_w};
+                    c->order, c->sort_count, c->sort_off,
+                    c->d_scal, c->d_res, c->d_mlp_w};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     field_free(*c);
@@ -285,11 +296,26 @@ extern "C" ep_status ep_plan_fillers(const ep_netlist *nl, const ep_params *p, i
     if (fw_out) *fw_out = fw;
     if (fh_out) *fh_out = fh;
     if (x && y) {
+        // uniform random corners, then stored in Morton order of their bins so
+        // that consecutive fillers are spatial neighbours (density scatter tiles)
         uint64_t rs = p->filler_seed;
+        std::vector<std::pair<uint64_t, int64_t>> key((size_t)nf);
+        std::vector<float> fx((size_t)nf), fy((size_t)nf);
+        const double bw = ((double)nl->xh - nl->xl) / p->bins_x, bh = ((double)nl->yh - nl->yl) / p->bins_y;
+        for (int64_t k = 0; k < nf; ++k) {
+            fx[k] = (float)(nl->xl + rng_uniform(rs) * ((double)nl->xh - nl->xl - fw));
+            fy[k] = (float)(nl->yl + rng_uniform(rs) * ((double)nl->yh - nl->yl - fh));
+            const uint32_t i = (uint32_t)std::min(65535.0, std::max(0.0, (fx[k] + 0.5 * fw - nl->xl) / bw));
+            const uint32_t j = (uint32_t)std::min(65535.0, std::max(0.0, (fy[k] + 0.5 * fh - nl->yl) / bh));
+            uint64_t m = 0;
+            for (int b = 0; b < 16; ++b) m |= (uint64_t)((i >> b) & 1u) << (2 * b) | (uint64_t)((j >> b) & 1u) << (2 * b + 1);
+            key[(size_t)k] = {m, k};
+        }
+        std::stable_sort(key.begin(), key.end());
         const int64_t n = std::min<int64_t>(nf, cap);
         for (int64_t k = 0; k < n; ++k) {
-            x[k] = (float)(nl->xl + rng_uniform(rs) * ((double)nl->xh - nl->xl - fw));
-            y[k] = (float)(nl->yl + rng_uniform(rs) * ((double)nl->yh - nl->yl - fh));
+            x[k] = fx[(size_t)key[(size_t)k].second];
+            y[k] = fy[(size_t)key[(size_t)k].second];
         }
     }
     return EP_OK;
@@ -374,6 +400,20 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
         while (lane_classes[b] != g) ++b;
         cls[b].push_back(n);
     }
+    // within a class, order nets by their lowest cell index: nets on nearby
+    // cells then sit near each other, so the pin -> cell gathers of the WA
+    // kernel and the cell -> pin gathers of the per-cell sum stay cache-local
+    {
+        std::vector<int32_t> key(N, 0);
+        for (int n = 0; n < N; ++n) {
+            int32_t m = INT32_MAX;
+            for (int q = nl->net_start[n]; q < nl->net_start[n + 1]; ++q) m = std::min(m, nl->pin_cell[q]);
+            key[n] = m;
+        }
+        for (int b = 0; b < kMaxBuckets; ++b)
+            std::stable_sort(cls[b].begin(), cls[b].end(),
+                             [&](int32_t a2, int32_t b2) { return key[a2] < key[b2]; });
+    }
     std::vector<int32_t> net_start(N + 1), pin_cell(nl->n_pins);
     std::vector<float2> pin_off(nl->n_pins);
     c->net_perm.clear();
@@ -404,7 +444,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     }
     net_start[N] = pin_w;
     c->wa_blocks = blk;
-    c->wa_part_blocks = blk;
+    c->wa_part_blocks = std::min(blk, kWaGridCap);
 
     // ---- cell -> pin CSR (movable cells)
     std::vector<int32_t> cps(c->n_mov + 1, 0);
@@ -427,7 +467,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     }
     const size_t nb = (size_t)c->Mx * c->My;
     c->ovf_blocks = (int)std::min<size_t>(1024, (nb + kBlock * 4 - 1) / (kBlock * 4));
-    c->cell_blocks = (int)std::min<int64_t>(2048, ((int64_t)c->n_place + kBlock - 1) / kBlock);
+    c->cell_blocks = (int)std::min<int64_t>(kCellGridCap, ((int64_t)c->n_place + kBlock - 1) / kBlock);
     c->part_wa_off = 0;
     c->part_ovf_off = 2 * c->wa_part_blocks;
     c->part_cell_off = c->part_ovf_off + c->ovf_blocks;
@@ -457,6 +497,9 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     EP_TRY(dalloc(&c->map_field, nb));
     EP_TRY(dalloc(&c->map_mov, nb));
     EP_TRY(dalloc(&c->map_fix, nb));
+    EP_TRY(dalloc(&c->order, std::max(1, c->n_place)));
+    EP_TRY(dalloc(&c->sort_count, kSortTiles));
+    EP_TRY(dalloc(&c->sort_off, kSortTiles));
     EP_TRY(dalloc(&c->ex, nb));
     EP_TRY(dalloc(&c->ey, nb));
     EP_TRY(dalloc(&c->tA, 2 * nb));
@@ -489,7 +532,8 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     if (hipMemset(c->pin_grad, 0, sizeof(float2) * std::max(1, nl->n_pins)) != hipSuccess ||
         hipMemset(c->ex, 0, sizeof(float) * nb) != hipSuccess ||
         hipMemset(c->ey, 0, sizeof(float) * nb) != hipSuccess ||
-        hipMemset(c->part, 0, sizeof(double) * c->part_total) != hipSuccess) {
+        hipMemset(c->part, 0, sizeof(double) * c->part_total) != hipSuccess ||
+        hipMemset(c->sort_count, 0, sizeof(int32_t) * kSortTiles) != hipSuccess) {
         destroy_ctx(c);
         return fail(EP_EDEVICE, "ep_create: hipMemset failed");
     }
@@ -591,6 +635,7 @@ extern "C" ep_status ep_get_positions(ep_ctx *c, float *x, float *y)
 static ep_status run_density(ep_ctx *c)
 {
     launch_fill_maps(*c, c->stream);
+    launch_sort(*c, c->V[c->cur], c->stream);
     launch_density(*c, c->V[c->cur], c->stream);
     launch_overflow(*c, c->stream);
     launch_finalize(*c, c->stream);
@@ -622,8 +667,8 @@ extern "C" ep_status ep_get_density(ep_ctx *c, float *field_map, float *mov_map,
 extern "C" ep_status ep_field(ep_ctx *c, int32_t precision)
 {
     if (!c) return fail(EP_EINVAL, "ep_field: null context");
-    if (precision < EP_PREC_FP32 || precision > EP_PREC_FP16X3)
-        return fail(EP_EINVAL, "ep_field: unknown precision %d", precision);
+    ep_status pst = check_precision(precision, "ep_field");
+    if (pst != EP_OK) return pst;
     launch_field(*c, precision, c->stream);
     ep_status st = check_launch();
     if (st != EP_OK) return st;
@@ -748,8 +793,8 @@ extern "C" ep_status ep_overflow(ep_ctx *c, double *overflow)
 extern "C" ep_status ep_set_precision(ep_ctx *c, int32_t precision)
 {
     if (!c) return fail(EP_EINVAL, "ep_set_precision: null context");
-    if (precision < EP_PREC_FP32 || precision > EP_PREC_FP16X3)
-        return fail(EP_EINVAL, "ep_set_precision: unknown precision %d", precision);
+    ep_status pst = check_precision(precision, "ep_set_precision");
+    if (pst != EP_OK) return pst;
     if (precision != c->prm.precision) {
         c->prm.precision = precision;
         for (int i = 0; i < 2; ++i) {
@@ -909,6 +954,7 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
     push_scalars(*c, c->stream);
     // field input must be a real map: build it once at the current positions
     launch_fill_maps(*c, c->stream);
+    launch_sort(*c, c->V[a], c->stream);
     launch_density(*c, c->V[a], c->stream);
     hipEvent_t e0, e1;
     EP_HIP(hipEventCreate(&e0));
@@ -921,7 +967,8 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
     const double M = c->Mx;
     const double gemm_flops = 12.0 * M * M * M * (c->prm.precision == EP_PREC_FP32 ? 1.0 : 3.0);
     Item items[] = {
-        {"density_scatter", 1, 16.0 * np + 8.0 * nb, 0},
+        {"cell_sort", 3, 2 * 16.0 * np + 8.0 * np + 8.0 * kSortTiles, 0},
+        {"density_scatter", 1, 4.0 * np + 16.0 * np + 8.0 * nb, 0},
         {"overflow", 1, 8.0 * nb, 0},
         {"wa_grad", 1, 4.0 * (N + 1) + 12.0 * P + 8.0 * c->n_all + 8.0 * P, 0},
         {"field_transforms", 4, 4.0 * 6 * nb + 4.0 * 6 * M * M, gemm_flops},
@@ -935,7 +982,8 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
         EP_HIP(hipEventRecord(e0, c->stream));
         for (int r = 0; r < reps; ++r) {
             const std::string nm = it.name;
-            if (nm == "density_scatter") launch_density(*c, c->V[a], c->stream);
+            if (nm == "cell_sort") launch_sort(*c, c->V[a], c->stream);
+            else if (nm == "density_scatter") launch_density(*c, c->V[a], c->stream);
             else if (nm == "overflow") launch_overflow(*c, c->stream);
             else if (nm == "wa_grad") launch_wa(*c, c->V[a], c->stream);
             else if (nm == "field_transforms") launch_field(*c, c->prm.precision, c->stream);
@@ -965,6 +1013,7 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
     *n = k;
     // leave the maps consistent with the current positions
     launch_fill_maps(*c, c->stream);
+    launch_sort(*c, c->V[a], c->stream);
     launch_density(*c, c->V[a], c->stream);
     return sync(*c);
 }

@@ -26,6 +26,13 @@
 namespace ep {
 
 constexpr int kBlock = 256;
+// Launch caps of the reducing kernels (bounds the partial sums to finalize).
+constexpr int kWaGridCap = 2048;
+constexpr int kCellGridCap = 1024;
+// Spatial counting sort of cells: kSortSide x kSortSide coarse tiles.
+constexpr int kSortSide = 64;
+constexpr int kSortTiles = kSortSide * kSortSide;
+constexpr int kSortCellsPerThread = 8;
 // Default HPWL reference of the lambda schedule: this many bin widths per net.
 constexpr double kHpwlRefPerNetBin = 0.05;
 constexpr int kMaxBuckets = 6;  // lanes per net: 2,4,8,16,32,64
@@ -67,6 +74,7 @@ struct Ctx;
 
 // ---- kernel launchers (defined in the .hip files) ----
 void launch_fill_maps(Ctx &c, hipStream_t s);
+void launch_sort(Ctx &c, const float2 *pos, hipStream_t s);
 void launch_density(Ctx &c, const float2 *pos, hipStream_t s);
 void launch_overflow(Ctx &c, hipStream_t s);
 void launch_wa(Ctx &c, const float2 *pos, hipStream_t s);
@@ -109,12 +117,11 @@ struct Ctx {
 
     // density + field
     float *map_field = nullptr, *map_mov = nullptr, *map_fix = nullptr;
+    int32_t *order = nullptr;                        // cells in spatial (coarse tile) order
+    int32_t *sort_count = nullptr, *sort_off = nullptr;  // kSortTiles each
     float *ex = nullptr, *ey = nullptr;
     float *tA = nullptr, *tB = nullptr;         // transform scratch, 2*Mx*My each
-    float *Cx = nullptr, *Sx_T = nullptr, *Cx_T = nullptr;  // Mx*Mx
-    float *Cy = nullptr, *Sy = nullptr, *Cy_T = nullptr;    // My*My
-    void *split_mats = nullptr;                 // split BF16/FP16 operand copies
-    void *split_scratch = nullptr;              // split data operands
+    void *split_mats = nullptr;                 // host handle: FP32 + split transform matrices
     int32_t field_precision_built = -1;
 
     // reductions

@@ -19,8 +19,8 @@ namespace ep {
 
 namespace {
 
-constexpr int kTile = 4096;     // bins per LDS-privatised tile (per map)
-constexpr int kCellsPerThread = 4;
+constexpr int kTile = 4096;             // bins per LDS-privatised tile (per map)
+constexpr int kDensCellsPerThread = 8;  // cells per density-scatter thread
 
 struct Geom {
     float xl, yl, xh, yh, bw, bh, sqrt2_bw, sqrt2_bh;
@@ -107,10 +107,11 @@ __device__ inline double wave_max(double v)
 }
 
 // Block-wide sum (and max) of NV doubles; thread 0 writes out[k].
-template <int NV, int NMAX>
+template <int NV, int NMAX, int NT = kBlock>
 __device__ inline void block_reduce(double (&v)[NV], double *out)
 {
-    __shared__ double red[kBlock / 64][NV];
+    constexpr int kWaves = NT / 64;
+    __shared__ double red[kWaves][NV];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = (k < NV - NMAX) ? wave_sum(v[k]) : wave_max(v[k]);
@@ -122,7 +123,7 @@ __device__ inline void block_reduce(double (&v)[NV], double *out)
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double a = red[0][k];
-            for (int w = 1; w < kBlock / 64; ++w)
+            for (int w = 1; w < kWaves; ++w)
                 a = (k < NV - NMAX) ? a + red[w][k] : fmax(a, red[w][k]);
             out[k] = a;
         }
@@ -140,11 +141,112 @@ __global__ void k_fill_maps(float *__restrict__ map_field, float *__restrict__ m
     }
 }
 
-// Density scatter.  Each block takes kBlock*kCellsPerThread consecutive cells;
-// if the union of their bin ranges fits in kTile bins the block accumulates
-// in LDS and flushes non-zero bins with one global atomic each, otherwise it
-// adds straight into the global maps.
-__global__ __launch_bounds__(kBlock) void k_density(const float2 *__restrict__ pos,
+// ---- spatial counting sort of the cells by coarse tile (64 x 64 tiles in
+// Morton order).  Any order gives the same maps; a spatially sorted order makes
+// each density block's cells share a small bin window, so the block can
+// accumulate in LDS and flush with coalesced atomics.
+
+__device__ inline int morton6(int i, int j)
+{
+    int m = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) m |= (((i >> b) & 1) << (2 * b)) | (((j >> b) & 1) << (2 * b + 1));
+    return m;
+}
+
+__device__ inline int cell_key(float2 p, float2 s, const GeomArgs &g, int Mx, int My)
+{
+    Rect r;
+    stretched_rect(p, s, g, Mx, My, r);
+    const int i = min(max(r.i0, 0), Mx - 1), j = min(max(r.j0, 0), My - 1);
+    return morton6((i * kSortSide) / Mx, (j * kSortSide) / My);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_count(const float2 *__restrict__ pos,
+                                                       const float2 *__restrict__ wh, int n_place,
+                                                       GeomArgs g, int Mx, int My,
+                                                       int32_t *__restrict__ count)
+{
+    __shared__ int hist[kSortTiles];
+    for (int t = threadIdx.x; t < kSortTiles; t += kBlock) hist[t] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kBlock * kSortCellsPerThread;
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k) {
+        const int idx = base + k * kBlock + threadIdx.x;
+        if (idx < n_place) atomicAdd(&hist[cell_key(pos[idx], wh[idx], g, Mx, My)], 1);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kSortTiles; t += kBlock)
+        if (hist[t]) atomicAdd(&count[t], hist[t]);
+}
+
+// Exclusive scan of the tile counts into offsets; clears the counts.
+__global__ __launch_bounds__(1024) void k_sort_scan(int32_t *__restrict__ count,
+                                                    int32_t *__restrict__ off)
+{
+    constexpr int per = kSortTiles / 1024;
+    __shared__ int part[1024];
+    int v[per], s = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        v[k] = count[threadIdx.x * per + k];
+        s += v[k];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // Hillis-Steele inclusive scan
+        const int add = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - s;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        off[threadIdx.x * per + k] = run;
+        run += v[k];
+        count[threadIdx.x * per + k] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(const float2 *__restrict__ pos,
+                                                         const float2 *__restrict__ wh, int n_place,
+                                                         GeomArgs g, int Mx, int My,
+                                                         int32_t *__restrict__ off,
+                                                         int32_t *__restrict__ order)
+{
+    __shared__ int hist[kSortTiles];
+    __shared__ int base_of[kSortTiles];
+    for (int t = threadIdx.x; t < kSortTiles; t += kBlock) hist[t] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kBlock * kSortCellsPerThread;
+    int key[kSortCellsPerThread], rank[kSortCellsPerThread];
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k) {
+        const int idx = base + k * kBlock + threadIdx.x;
+        key[k] = -1;
+        if (idx < n_place) {
+            key[k] = cell_key(pos[idx], wh[idx], g, Mx, My);
+            rank[k] = atomicAdd(&hist[key[k]], 1);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kSortTiles; t += kBlock)
+        if (hist[t]) base_of[t] = atomicAdd(&off[t], hist[t]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k)
+        if (key[k] >= 0) order[base_of[key[k]] + rank[k]] = base + k * kBlock + threadIdx.x;
+}
+
+// ---- density scatter.  Each block takes kBlock * kDensCellsPerThread cells of
+// the sorted order; if the union of their bin ranges fits in kTile bins the
+// block accumulates in LDS and flushes each non-zero bin with one global
+// atomic (coalesced along bin rows), otherwise it adds straight into the
+// global maps.
+__global__ __launch_bounds__(kBlock) void k_density(const int32_t *__restrict__ order,
+                                                    const float2 *__restrict__ pos,
                                                     const float2 *__restrict__ wh, int n_place,
                                                     int n_mov, GeomArgs g, int Mx, int My,
                                                     float *__restrict__ map_field,
@@ -152,27 +254,26 @@ __global__ __launch_bounds__(kBlock) void k_density(const float2 *__restrict__ p
 {
     __shared__ float tile[2][kTile];
     __shared__ int box[kBlock / 64][4];
-    const int base = blockIdx.x * kBlock * kCellsPerThread;
+    const int base = blockIdx.x * kBlock * kDensCellsPerThread;
     const float area_scale = g.v[4] * g.v[5];
-
     int bi0 = INT_MAX, bi1 = -1, bj0 = INT_MAX, bj1 = -1;
 #pragma unroll
-    for (int k = 0; k < kCellsPerThread; ++k) {
-        int c = base + k * kBlock + threadIdx.x;
-        if (c < n_place) {
-            float2 p = pos[c], s = wh[c];
-            Rect r;
-            stretched_rect(p, s, g, Mx, My, r);
+    for (int k = 0; k < kDensCellsPerThread; ++k) {
+        const int idx = base + k * kBlock + threadIdx.x;
+        if (idx >= n_place) break;
+        const int c = order[idx];
+        const float2 p = pos[c], s = wh[c];
+        Rect r;
+        stretched_rect(p, s, g, Mx, My, r);
+        if (r.i0 <= r.i1 && r.j0 <= r.j1) {
+            bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
+            bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
+        }
+        if (c < n_mov) {
+            exact_rect(p, s, g, Mx, My, r);
             if (r.i0 <= r.i1 && r.j0 <= r.j1) {
                 bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
                 bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
-            }
-            if (c < n_mov) {
-                exact_rect(p, s, g, Mx, My, r);
-                if (r.i0 <= r.i1 && r.j0 <= r.j1) {
-                    bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
-                    bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
-                }
             }
         }
     }
@@ -206,20 +307,21 @@ __global__ __launch_bounds__(kBlock) void k_density(const float2 *__restrict__ p
         __syncthreads();
     }
 #pragma unroll 1
-    for (int k = 0; k < kCellsPerThread; ++k) {
-        int c = base + k * kBlock + threadIdx.x;
-        if (c >= n_place) break;
-        float2 p = pos[c], s = wh[c];
-        Rect r;
-        stretched_rect(p, s, g, Mx, My, r);
-        const float sc = r.ratio * area_scale;
-        for (int i = r.i0; i <= r.i1; ++i) {
-            float ox = ovl(r.tx0, r.tx1, i);
+    for (int k = 0; k < kDensCellsPerThread; ++k) {
+        const int idx = base + k * kBlock + threadIdx.x;
+        if (idx >= n_place) break;
+        const int c = order[idx];
+        const float2 p = pos[c], s = wh[c];
+        Rect rs;
+        stretched_rect(p, s, g, Mx, My, rs);
+        const float sc = rs.ratio * area_scale;
+        for (int i = rs.i0; i <= rs.i1; ++i) {
+            const float ox = ovl(rs.tx0, rs.tx1, i);
             if (ox <= 0.0f) continue;
-            for (int j = r.j0; j <= r.j1; ++j) {
-                float oy = ovl(r.ty0, r.ty1, j);
+            for (int j = rs.j0; j <= rs.j1; ++j) {
+                const float oy = ovl(rs.ty0, rs.ty1, j);
                 if (oy <= 0.0f) continue;
-                float v = ox * oy * sc;
+                const float v = ox * oy * sc;
                 if (lds)
                     atomicAdd(&tile[0][(i - bi0) * tw + (j - bj0)], v);
                 else
@@ -227,14 +329,15 @@ __global__ __launch_bounds__(kBlock) void k_density(const float2 *__restrict__ p
             }
         }
         if (c < n_mov) {
-            exact_rect(p, s, g, Mx, My, r);
-            for (int i = r.i0; i <= r.i1; ++i) {
-                float ox = ovl(r.tx0, r.tx1, i);
+            Rect re;
+            exact_rect(p, s, g, Mx, My, re);
+            for (int i = re.i0; i <= re.i1; ++i) {
+                const float ox = ovl(re.tx0, re.tx1, i);
                 if (ox <= 0.0f) continue;
-                for (int j = r.j0; j <= r.j1; ++j) {
-                    float oy = ovl(r.ty0, r.ty1, j);
+                for (int j = re.j0; j <= re.j1; ++j) {
+                    const float oy = ovl(re.ty0, re.ty1, j);
                     if (oy <= 0.0f) continue;
-                    float v = ox * oy * area_scale;
+                    const float v = ox * oy * area_scale;
                     if (lds)
                         atomicAdd(&tile[1][(i - bi0) * tw + (j - bj0)], v);
                     else
@@ -407,21 +510,24 @@ __global__ __launch_bounds__(kBlock) void k_wa(BucketTable bt, const int32_t *__
                                                const float2 *__restrict__ pos,
                                                float2 *__restrict__ pin_grad,
                                                const DevScalars *__restrict__ sc,
-                                               double *__restrict__ part)
+                                               int total_blocks, double *__restrict__ part)
 {
-    int b = 0;
-    while (b + 1 < bt.n && (int)blockIdx.x >= bt.b[b + 1].blk0) ++b;
-    const Bucket bk = bt.b[b];
-    const int blk = blockIdx.x - bk.blk0;
     const float ginv = sc->gamma_inv;
     double hp = 0.0, wa = 0.0;
-    switch (bk.lanes) {
-    case 2: wa_bucket<2>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
-    case 4: wa_bucket<4>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
-    case 8: wa_bucket<8>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
-    case 16: wa_bucket<16>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
-    case 32: wa_bucket<32>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
-    default: wa_bucket<64>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+    // virtual blocks: the launch is capped, each block walks several tiles of nets
+    for (int vb = blockIdx.x; vb < total_blocks; vb += gridDim.x) {
+        int b = 0;
+        while (b + 1 < bt.n && vb >= bt.b[b + 1].blk0) ++b;
+        const Bucket bk = bt.b[b];
+        const int blk = vb - bk.blk0;
+        switch (bk.lanes) {
+        case 2: wa_bucket<2>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+        case 4: wa_bucket<4>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+        case 8: wa_bucket<8>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+        case 16: wa_bucket<16>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+        case 32: wa_bucket<32>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+        default: wa_bucket<64>(bk, blk, net_start, pin_cell, pin_off, pos, pin_grad, ginv, hp, wa); break;
+        }
     }
     double v[2] = {hp, wa};
     block_reduce<2, 0>(v, part + 2 * blockIdx.x);
@@ -499,24 +605,25 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
 }
 
 // Deterministic final reduction of all partial sums (fixed order).
-__global__ __launch_bounds__(kBlock) void k_finalize(const double *__restrict__ part, int wa_off,
-                                                     int wa_blocks, int ovf_off, int ovf_blocks,
-                                                     int cell_off, int cell_blocks, double mov_area,
-                                                     DevResult *__restrict__ res)
+constexpr int kFinalizeThreads = 1024;
+
+__global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
+    const double *__restrict__ part, int wa_off, int wa_blocks, int ovf_off, int ovf_blocks,
+    int cell_off, int cell_blocks, double mov_area, DevResult *__restrict__ res)
 {
     double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < wa_blocks; b += kBlock) {
+    for (int b = threadIdx.x; b < wa_blocks; b += kFinalizeThreads) {
         v[0] += part[wa_off + 2 * b];
         v[1] += part[wa_off + 2 * b + 1];
     }
-    for (int b = threadIdx.x; b < ovf_blocks; b += kBlock) v[2] += part[ovf_off + b];
-    for (int b = threadIdx.x; b < cell_blocks; b += kBlock) {
+    for (int b = threadIdx.x; b < ovf_blocks; b += kFinalizeThreads) v[2] += part[ovf_off + b];
+    for (int b = threadIdx.x; b < cell_blocks; b += kFinalizeThreads) {
         const double *q = part + cell_off + 7 * b;
         v[3] += q[0]; v[4] += q[1]; v[5] += q[2]; v[6] += q[3]; v[7] += q[4]; v[8] += q[5];
         v[9] = fmax(v[9], q[6]);
     }
     __shared__ double out[10];
-    block_reduce<10, 1>(v, out);
+    block_reduce<10, 1, kFinalizeThreads>(v, out);
     if (threadIdx.x == 0) {
         res->hpwl = out[0];
         res->wa = out[1];
@@ -617,11 +724,21 @@ void launch_fill_maps(Ctx &c, hipStream_t s)
                        c.map_field, c.map_mov, c.map_fix, c.prm.target_density, nb);
 }
 
+void launch_sort(Ctx &c, const float2 *pos, hipStream_t s)
+{
+    const int blocks = grid_for(c.n_place, kBlock * kSortCellsPerThread, INT_MAX);
+    hipLaunchKernelGGL(k_sort_count, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
+                       geom_args(c), c.Mx, c.My, c.sort_count);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, c.sort_count, c.sort_off);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
+                       geom_args(c), c.Mx, c.My, c.sort_off, c.order);
+}
+
 void launch_density(Ctx &c, const float2 *pos, hipStream_t s)
 {
-    const int blocks = grid_for(c.n_place, kBlock * kCellsPerThread, INT_MAX);
-    hipLaunchKernelGGL(k_density, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place, c.n_mov,
-                       geom_args(c), c.Mx, c.My, c.map_field, c.map_mov);
+    const int blocks = grid_for(c.n_place, kBlock * kDensCellsPerThread, INT_MAX);
+    hipLaunchKernelGGL(k_density, dim3(blocks), dim3(kBlock), 0, s, c.order, pos, c.wh, c.n_place,
+                       c.n_mov, geom_args(c), c.Mx, c.My, c.map_field, c.map_mov);
 }
 
 void launch_overflow(Ctx &c, hipStream_t s)
@@ -635,8 +752,9 @@ void launch_overflow(Ctx &c, hipStream_t s)
 void launch_wa(Ctx &c, const float2 *pos, hipStream_t s)
 {
     if (c.wa_blocks == 0) return;
-    hipLaunchKernelGGL(k_wa, dim3(c.wa_blocks), dim3(kBlock), 0, s, c.buckets, c.net_start,
-                       c.pin_cell, c.pin_off, pos, c.pin_grad, c.d_scal, c.part + c.part_wa_off);
+    hipLaunchKernelGGL(k_wa, dim3(c.wa_part_blocks), dim3(kBlock), 0, s, c.buckets, c.net_start,
+                       c.pin_cell, c.pin_off, pos, c.pin_grad, c.d_scal, c.wa_blocks,
+                       c.part + c.part_wa_off);
 }
 
 void launch_cell_grad(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref,
@@ -650,7 +768,7 @@ void launch_cell_grad(Ctx &c, const float2 *pos, const float2 *vref, const float
 
 void launch_finalize(Ctx &c, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, c.part, c.part_wa_off,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, s, c.part, c.part_wa_off,
                        c.wa_part_blocks, c.part_ovf_off, c.ovf_blocks, c.part_cell_off,
                        c.cell_blocks, c.mov_area, c.d_res);
 }

@@ -135,7 +135,7 @@ def main():
     ap.add_argument("--workload", default="stdcell_2m")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="plain,mixed_learned")
+    ap.add_argument("--forms", default="plain,mixed,mixed_learned")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -152,6 +152,7 @@ def main():
 
     forms_all = {
         "plain": dict(name="plain", precision=api.EP_PREC_FP32, learned_step=0),
+        "mixed": dict(name="mixed", precision=api.EP_PREC_BF16X3, learned_step=0),
         "mixed_learned": dict(name="mixed_learned", precision=api.EP_PREC_BF16X3, learned_step=1),
     }
     results, placers = {}, {}
@@ -170,7 +171,7 @@ def main():
                              iterations=runs[-1]["iterations"], evals=runs[-1]["evals"],
                              hpwl=runs[-1]["hpwl"], overflow=round(runs[-1]["overflow"], 4))
 
-    head = "mixed_learned" if "mixed_learned" in results else "plain"
+    head = next((f for f in ("mixed_learned", "mixed", "plain") if f in results), "plain")
     plain = results["plain"]
     out = dict(metric=METRIC, value=results[head]["seconds"], unit="s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(results[head]["seconds"] * 1000, 2),
