@@ -87,7 +87,17 @@ EXPORTS = [
     "ep_cell_bins", "ep_hpwl", "ep_overflow", "ep_init_optimizer", "ep_step", "ep_run",
     "ep_set_precision", "ep_profile", "ep_take_step_records", "ep_set_step_model", "ep_wa_packet",
     "ep_gen_default_params", "ep_gen_create", "ep_gen_free",
+    "ep_mlp_default_train_params", "ep_mlp_train", "ep_mlp_forward", "ep_mlp_grad",
 ]
+
+MLP_IN, MLP_HID = 16, 64
+MLP_NWEIGHTS = 34 + MLP_HID * MLP_IN + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID + 1
+
+
+class EpMlpTrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("seed", C.c_uint64)]
 
 
 class EplaceError(RuntimeError):
@@ -138,6 +148,11 @@ def _load():
     L.ep_gen_create.argtypes = [C.POINTER(EpGenParams), C.POINTER(C.POINTER(EpDesign))]
     L.ep_gen_free.argtypes = [C.POINTER(EpDesign)]
     L.ep_gen_free.restype = None
+    L.ep_mlp_default_train_params.argtypes = [C.c_void_p]
+    L.ep_mlp_default_train_params.restype = None
+    L.ep_mlp_train.argtypes = [_fp, _fp, C.c_int32, C.c_void_p, _fp, _fp]
+    L.ep_mlp_forward.argtypes = [_fp, _fp, C.c_int32, _fp]
+    L.ep_mlp_grad.argtypes = [_fp, _fp, _fp, C.c_int32, _fp, _fp]
     return L
 
 

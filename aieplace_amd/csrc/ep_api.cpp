@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/eplace_mlp.h"
 #include "ep_ctx.h"
 
 namespace ep {
@@ -145,6 +146,7 @@ ep_status trial_eval(Ctx &c)
             hipGraph_t g = nullptr;
             EP_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
             push_scalars(c, c.stream);
+            launch_mlp_propose(c, c.stream);
             launch_trial(c, c.V[a], c.U[a], c.G[a], c.V[b], c.U[b], c.stream);
             enqueue_eval(c, c.V[b], c.V[a], c.G[a], c.G[b], nullptr, nullptr, c.stream);
             EP_HIP(hipStreamEndCapture(c.stream, &g));
@@ -156,6 +158,7 @@ ep_status trial_eval(Ctx &c)
         EP_HIP(hipGraphLaunch(c.graph_exec[a], c.stream));
     } else {
         push_scalars(c, c.stream);
+        launch_mlp_propose(c, c.stream);
         launch_trial(c, c.V[a], c.U[a], c.G[a], c.V[b], c.U[b], c.stream);
         enqueue_eval(c, c.V[b], c.V[a], c.G[a], c.G[b], nullptr, nullptr, c.stream);
         ep_status st = check_launch();
@@ -223,7 +226,6 @@ static ep_status validate(const ep_netlist *nl, const ep_params *p)
     if (p->max_backtrack < 1) return fail(EP_EINVAL, "ep_create: max_backtrack must be >= 1");
     ep_status pst = check_precision(p->precision, "ep_create");
     if (pst != EP_OK) return pst;
-    if (p->learned_step) return fail(EP_ENOTSUP, "ep_create: learned step not available in this build");
     const int n_all = nl->n_mov + nl->n_fix;
     for (int k = 0; k < n_all; ++k) {
         if (!(nl->w[k] > 0.f) || !(nl->h[k] > 0.f) || !std::isfinite(nl->w[k]) || !std::isfinite(nl->h[k]))
@@ -852,7 +854,60 @@ extern "C" ep_status ep_init_optimizer(ep_ctx *c)
     if (!(c->alpha > 0) || !std::isfinite(c->alpha)) c->alpha = eps > 0 ? eps : 1e-3;
     c->ak = 1.0;
     c->iteration = 0;
+    c->h_alpha1 = c->h_alpha2 = c->alpha;
+    c->h_prev_tau = c->last_tau;
+    c->h_prev_hpwl = c->prev_hpwl;
+    c->h_ratio = 1.0;
+    c->h_gnorm = std::sqrt(c->h_res->gnorm2);
+    c->h_dv = num;
+    c->h_dg = den;
+    c->h_sum_ratio = sf > 0 ? sw / sf : 1.0;
+    c->h_evals = 1;
     c->opt_ready = true;
+    return EP_OK;
+}
+
+// Step-model features of the coming iteration (layout in include/eplace_mlp.h).
+static void step_features(const ep_ctx *c, double coef, float *f)
+{
+    auto lg = [](double v) { return (float)std::log10(std::max(v, 1e-30)); };
+    f[0] = lg(c->alpha);
+    f[1] = lg(c->h_alpha1);
+    f[2] = lg(c->h_alpha2);
+    f[3] = (float)c->last_tau;
+    f[4] = (float)(c->last_tau - c->h_prev_tau);
+    f[5] = lg(c->lambda);
+    f[6] = lg(c->gamma / c->bw);
+    f[7] = (float)((c->last_hpwl - c->h_prev_hpwl) / std::max(c->last_hpwl, 1e-30));
+    f[8] = lg(c->h_gnorm);
+    f[9] = lg(c->h_dv);
+    f[10] = lg(c->h_dg);
+    f[11] = (float)coef;
+    f[12] = (float)c->h_evals;
+    f[13] = lg(c->h_sum_ratio);
+    f[14] = lg(c->h_ratio);
+    f[15] = (float)(c->iteration / 1000.0);
+}
+
+// One trial with step `alpha` (model proposal when use_model); returns alpha_hat
+// and the step actually used.
+static ep_status one_trial(ep_ctx *c, double alpha, double coef, int use_model, double *alpha_hat,
+                           double *alpha_used)
+{
+    c->h_scal->alpha = (float)alpha;
+    c->h_scal->coef = (float)coef;
+    c->h_scal->gamma_inv = (float)(1.0 / c->gamma);
+    c->h_scal->lambda = (float)c->lambda;
+    c->h_scal->use_model = use_model;
+    ep_status st = trial_eval(*c);
+    if (st != EP_OK) return st;
+    c->evals++;
+    if (c->h_res->nonfinite)
+        return fail(EP_ENAN, "ep_step: non-finite gradient at iteration %d", c->iteration);
+    const double used = c->h_res->alpha_used;
+    const double nn = std::sqrt(c->h_res->dv2), dd = std::sqrt(c->h_res->dg2);
+    *alpha_hat = dd > 0 ? nn / dd : used;
+    *alpha_used = used;
     return EP_OK;
 }
 
@@ -863,36 +918,59 @@ extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
     if (!c->opt_ready && (st = ep_init_optimizer(c)) != EP_OK) return st;
     const double ak1 = 0.5 * (1.0 + std::sqrt(4.0 * c->ak * c->ak + 1.0));
     const double coef = (c->ak - 1.0) / ak1;
-    double alpha = c->alpha, alpha_hat = alpha;
+    float feat[EP_MLP_IN];
+    step_features(c, coef, feat);
+    std::memcpy(c->h_scal->feat, feat, sizeof(feat));
+    double alpha_hat = c->alpha, used = c->alpha;
     int trials = 0;
-    for (int t = 0; t < c->prm.max_backtrack; ++t) {
-        c->h_scal->alpha = (float)alpha;
-        c->h_scal->coef = (float)coef;
-        c->h_scal->gamma_inv = (float)(1.0 / c->gamma);
-        c->h_scal->lambda = (float)c->lambda;
-        if ((st = trial_eval(*c)) != EP_OK) return st;
+    bool accepted = false, learned_used = false, learned_ok = false;
+    // learned proposal first, verified by the backtracking condition
+    if (c->prm.learned_step && c->mlp_loaded) {
+        learned_used = true;
+        c->learned_proposals++;
+        if ((st = one_trial(c, c->alpha, coef, 1, &alpha_hat, &used)) != EP_OK) return st;
         ++trials;
-        c->evals++;
-        if (c->h_res->nonfinite)
-            return fail(EP_ENAN, "ep_step: non-finite gradient at iteration %d", c->iteration);
-        const double nn = std::sqrt(c->h_res->dv2), dd = std::sqrt(c->h_res->dg2);
-        alpha_hat = dd > 0 ? nn / dd : alpha;
-        if (alpha_hat >= 0.95 * alpha || t == c->prm.max_backtrack - 1) break;
-        alpha = alpha_hat;
+        if (std::isfinite(alpha_hat) && alpha_hat >= 0.95 * used) {
+            accepted = learned_ok = true;
+            c->learned_accepted++;
+        }
+    }
+    // plain backtracking search from the Lipschitz step (also the fallback)
+    double alpha = c->alpha;
+    for (int t = 0; !accepted && t < c->prm.max_backtrack; ++t) {
+        if ((st = one_trial(c, alpha, coef, 0, &alpha_hat, &used)) != EP_OK) return st;
+        ++trials;
+        if (alpha_hat >= 0.95 * used || t == c->prm.max_backtrack - 1) accepted = true;
+        else alpha = alpha_hat;
+    }
+    // record (features, accepted step) for training the step model
+    if (c->rec_target.size() < kMaxStepRecords) {
+        c->rec_feat.insert(c->rec_feat.end(), feat, feat + EP_MLP_IN);
+        c->rec_target.push_back((float)std::log10(std::max(used, 1e-30)));
     }
     // accept: the trial buffers become the current ones
     c->cur ^= 1;
     c->alpha = alpha_hat;
     c->ak = ak1;
     const double hp = c->h_res->hpwl, tau = c->h_res->tau;
+    c->h_alpha2 = c->h_alpha1;
+    c->h_alpha1 = used;
+    c->h_prev_tau = c->last_tau;
+    c->h_prev_hpwl = c->last_hpwl;
+    c->h_ratio = alpha_hat / std::max(used, 1e-30);
+    c->h_gnorm = std::sqrt(c->h_res->gnorm2);
+    c->h_dv = std::sqrt(c->h_res->dv2);
+    c->h_dg = std::sqrt(c->h_res->dg2);
+    c->h_sum_ratio = c->h_res->sum_f > 0 ? c->h_res->sum_gw / c->h_res->sum_f : 1.0;
+    c->h_evals = trials;
     c->last_hpwl = hp;
     c->last_tau = tau;
     c->last_wa = c->h_res->wa;
     if (info) {
         info->iteration = c->iteration;
         info->evals = trials;
-        info->learned_used = 0;
-        info->learned_accepted = 0;
+        info->learned_used = learned_used ? 1 : 0;
+        info->learned_accepted = learned_ok ? 1 : 0;
         info->hpwl = hp;
         info->overflow = tau;
         info->lambda = c->lambda;
@@ -1022,15 +1100,39 @@ extern "C" ep_status ep_take_step_records(ep_ctx *c, float *features, float *log
                                           int32_t *n)
 {
     if (!c || !n) return fail(EP_EINVAL, "ep_take_step_records: null argument");
-    (void)features; (void)log_step; (void)cap;
-    *n = 0;
+    const int32_t have = (int32_t)c->rec_target.size();
+    *n = have;
+    if (!features || !log_step) return EP_OK;  // query only
+    const int32_t k = std::min(have, std::max(cap, 0));
+    std::memcpy(features, c->rec_feat.data(), sizeof(float) * (size_t)k * EP_MLP_IN);
+    std::memcpy(log_step, c->rec_target.data(), sizeof(float) * (size_t)k);
+    *n = k;
+    c->rec_feat.clear();
+    c->rec_target.clear();
     return EP_OK;
 }
 
 extern "C" ep_status ep_set_step_model(ep_ctx *c, const float *w, int32_t n)
 {
-    (void)c; (void)w; (void)n;
-    return fail(EP_ENOTSUP, "ep_set_step_model: learned step not built yet");
+    if (!c || !w) return fail(EP_EINVAL, "ep_set_step_model: null argument");
+    if (n != EP_MLP_NWEIGHTS)
+        return fail(EP_EINVAL, "ep_set_step_model: expected %d weights, got %d", EP_MLP_NWEIGHTS, n);
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(w[i])) return fail(EP_ENAN, "ep_set_step_model: non-finite weight %d", i);
+    for (int i = 16; i < 34; ++i)
+        if (i != 32 && !(w[i] > 0.0f)) return fail(EP_EINVAL, "ep_set_step_model: normalisation std must be > 0");
+    if (!c->d_mlp_w) {
+        ep_status st = dalloc(&c->d_mlp_w, (size_t)EP_MLP_NWEIGHTS);
+        if (st != EP_OK) return st;
+    }
+    EP_HIP(hipMemcpy(c->d_mlp_w, w, sizeof(float) * n, hipMemcpyHostToDevice));
+    c->mlp_loaded = true;
+    for (int i = 0; i < 2; ++i) {  // captured graphs must now include the proposal kernel
+        if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
+        c->graph_exec[i] = nullptr;
+        c->graph_built[i] = false;
+    }
+    return EP_OK;
 }
 
 extern "C" ep_status ep_wa_packet(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out)

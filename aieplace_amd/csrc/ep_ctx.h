@@ -33,6 +33,8 @@ constexpr int kCellGridCap = 1024;
 constexpr int kSortSide = 64;
 constexpr int kSortTiles = kSortSide * kSortSide;
 constexpr int kSortCellsPerThread = 8;
+// Step-model records kept per context between ep_take_step_records calls.
+constexpr size_t kMaxStepRecords = (size_t)1 << 21;
 // Default HPWL reference of the lambda schedule: this many bin widths per net.
 constexpr double kHpwlRefPerNetBin = 0.05;
 constexpr int kMaxBuckets = 6;  // lanes per net: 2,4,8,16,32,64
@@ -56,7 +58,9 @@ struct DevScalars {
     float gamma_inv;  // 1/gamma
     float lambda;     // density weight
     float probe_eps;  // displacement factor of the alpha_0 probe
-    int32_t pad[3];
+    int32_t use_model;  // 1: the MLP proposal kernel overwrites alpha
+    int32_t pad[2];
+    float feat[16];     // step-model features of the current iteration
 };
 
 // Reductions of one gradient evaluation (FP64), written by the finalize kernel.
@@ -67,7 +71,7 @@ struct DevResult {
     double gmax;            // max |g| component
     double gnorm2;          // |g'|^2
     int64_t nonfinite;      // count of non-finite gradient components
-    double pad;
+    double alpha_used;      // step length the trial actually used
 };
 
 struct Ctx;
@@ -85,6 +89,7 @@ void launch_finalize(Ctx &c, hipStream_t s);
 void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, float2 *v_new,
                   float2 *u_new, hipStream_t s);
 void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s);
+void launch_mlp_propose(Ctx &c, hipStream_t s);
 void launch_cell_bins(Ctx &c, const float2 *pos, int stretched, int32_t *out, hipStream_t s);
 ep_status field_init(Ctx &c);   // builds transform matrices
 void field_free(Ctx &c);
@@ -145,9 +150,11 @@ struct Ctx {
     // learned step (MLP proposal)
     float *d_mlp_w = nullptr;
     bool mlp_loaded = false;
-    std::vector<float> mlp_w_host;
-    std::vector<float> rec_feat, rec_target;   // accepted-step records
-    double feat_hist[8] = {0};                  // previous-iteration scalars
+    std::vector<float> rec_feat, rec_target;   // accepted-step records (features, log10 step)
+    // per-iteration history feeding the step-model features
+    double h_alpha1 = 0, h_alpha2 = 0, h_prev_tau = 0, h_prev_hpwl = 0, h_ratio = 1;
+    double h_gnorm = 0, h_dv = 0, h_dg = 0, h_sum_ratio = 1;
+    int32_t h_evals = 1;
     int32_t learned_proposals = 0, learned_accepted = 0;
 };
 

@@ -609,7 +609,8 @@ constexpr int kFinalizeThreads = 1024;
 
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     const double *__restrict__ part, int wa_off, int wa_blocks, int ovf_off, int ovf_blocks,
-    int cell_off, int cell_blocks, double mov_area, DevResult *__restrict__ res)
+    int cell_off, int cell_blocks, double mov_area, const DevScalars *__restrict__ sc,
+    DevResult *__restrict__ res)
 {
     double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < wa_blocks; b += kFinalizeThreads) {
@@ -636,6 +637,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
         res->gnorm2 = out[7];
         res->nonfinite = (int64_t)out[8];
         res->gmax = out[9];
+        res->alpha_used = sc->alpha;
     }
 }
 
@@ -770,7 +772,7 @@ void launch_finalize(Ctx &c, hipStream_t s)
 {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, s, c.part, c.part_wa_off,
                        c.wa_part_blocks, c.part_ovf_off, c.ovf_blocks, c.part_cell_off,
-                       c.cell_blocks, c.mov_area, c.d_res);
+                       c.cell_blocks, c.mov_area, c.d_scal, c.d_res);
 }
 
 void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, float2 *v_new,

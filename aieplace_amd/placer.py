@@ -140,6 +140,27 @@ class Placer:
         api.check(api.lib.ep_overflow(self._ctx, C.byref(v)))
         return v.value
 
+    # ------------------------------------------------------------ learned step
+    def set_step_model(self, weights):
+        w = np.ascontiguousarray(weights, np.float32)
+        api.check(api.lib.ep_set_step_model(self._ctx, api.fptr(w), len(w)))
+
+    def take_step_records(self):
+        """(features[n,16], log10 accepted step[n]) recorded since the last call."""
+        n = C.c_int32()
+        api.check(api.lib.ep_take_step_records(self._ctx, None, None, 0, C.byref(n)))
+        f = np.zeros((max(n.value, 1), api.MLP_IN), np.float32)
+        t = np.zeros(max(n.value, 1), np.float32)
+        api.check(api.lib.ep_take_step_records(self._ctx, api.fptr(f), api.fptr(t), n.value, C.byref(n)))
+        return f[:n.value], t[:n.value]
+
+    def profile(self, reps: int = 20):
+        stats = (api.EpKernelStat * 16)()
+        n = C.c_int32()
+        api.check(api.lib.ep_profile(self._ctx, int(reps), stats, 16, C.byref(n)))
+        return [dict(name=s.name.decode(), launches=s.launches, avg_us=s.avg_us, bytes=s.bytes,
+                     flops=s.flops) for s in stats[:n.value]]
+
     # ------------------------------------------------------------ optimiser
     def initializePlacement(self):
         api.check(api.lib.ep_init_optimizer(self._ctx))

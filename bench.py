@@ -38,7 +38,15 @@ def run_form(A, nl, cfg, form, steps, warmup):
     kw = dict(precision=form.get("precision", api.EP_PREC_FP32), use_graph=form.get("use_graph", 1),
               learned_step=form.get("learned_step", 0))
     p = A.placer_params(cfg, **kw)
+    weights = None
+    if kw["learned_step"]:
+        from aieplace_amd import learned
+        if not os.path.exists(learned.DEFAULT_WEIGHTS):
+            raise api.EplaceError(api.EP_ENOTSUP, f"no trained step model at {learned.DEFAULT_WEIGHTS}")
+        weights = learned.load()
     pl = A.Placer(nl, p)
+    if weights is not None:
+        pl.set_step_model(weights)
     x0, y0 = pl.get_positions()
     for w in range(warmup):
         pl.set_positions(x0, y0)
@@ -170,6 +178,11 @@ def main():
         results[name] = dict(seconds=round(float(np.mean(sec)), 4), runs=[round(s, 4) for s in sec],
                              iterations=runs[-1]["iterations"], evals=runs[-1]["evals"],
                              hpwl=runs[-1]["hpwl"], overflow=round(runs[-1]["overflow"], 4))
+        if form.get("learned_step"):
+            r = runs[-1]
+            results[name]["learned_proposals"] = r["learned_proposals"]
+            results[name]["learned_acceptance"] = round(
+                r["learned_accepted"] / max(1, r["learned_proposals"]), 4)
 
     head = next((f for f in ("mixed_learned", "mixed", "plain") if f in results), "plain")
     plain = results["plain"]

@@ -818,3 +818,87 @@ int eo_place(const eo_design *d, const eo_params *p, double *x, double *y, eo_it
     ws_free(&w);
     return it;
 }
+
+/* ------------------------------------------------------------------ */
+/* step-length MLP (FP64 reference for the device training kernels)   */
+/* ------------------------------------------------------------------ */
+
+#define MI 16
+#define MH 64
+#define OFF_W1 34
+#define OFF_B1 (OFF_W1 + MH * MI)
+#define OFF_W2 (OFF_B1 + MH)
+#define OFF_B2 (OFF_W2 + MH * MH)
+#define OFF_W3 (OFF_B2 + MH)
+#define OFF_B3 (OFF_W3 + MH)
+#define MLP_N (OFF_B3 + 1)
+
+int eo_mlp_nweights(void) { return MLP_N; }
+
+static double gelu(double z) { return 0.5 * z * (1.0 + erf(z / M_SQRT2)); }
+static double gelu_d(double z)
+{
+    return 0.5 * (1.0 + erf(z / M_SQRT2)) + z * exp(-0.5 * z * z) / sqrt(2.0 * M_PI);
+}
+
+/* one sample: normalised input -> z1, a1, z2, a2, normalised output */
+static double mlp_sample(const double *w, const double *xin, double *xn, double *z1, double *a1,
+                         double *z2, double *a2)
+{
+    for (int i = 0; i < MI; ++i) xn[i] = (xin[i] - w[i]) / w[16 + i];
+    for (int j = 0; j < MH; ++j) {
+        double s = w[OFF_B1 + j];
+        for (int i = 0; i < MI; ++i) s += w[OFF_W1 + j * MI + i] * xn[i];
+        z1[j] = s;
+        a1[j] = gelu(s);
+    }
+    for (int j = 0; j < MH; ++j) {
+        double s = w[OFF_B2 + j];
+        for (int k = 0; k < MH; ++k) s += w[OFF_W2 + j * MH + k] * a1[k];
+        z2[j] = s;
+        a2[j] = gelu(s);
+    }
+    double o = w[OFF_B3];
+    for (int k = 0; k < MH; ++k) o += w[OFF_W3 + k] * a2[k];
+    return o;
+}
+
+void eo_mlp_forward(const double *w, const double *x, int n, double *out)
+{
+    double xn[MI], z1[MH], a1[MH], z2[MH], a2[MH];
+    for (int s = 0; s < n; ++s)
+        out[s] = mlp_sample(w, x + (size_t)s * MI, xn, z1, a1, z2, a2) * w[33] + w[32];
+}
+
+double eo_mlp_grad(const double *w, const double *x, const double *y, int n, double *grad)
+{
+    double xn[MI], z1[MH], a1[MH], z2[MH], a2[MH], d2[MH], d1[MH];
+    memset(grad, 0, sizeof(double) * MLP_N);
+    double loss = 0.0;
+    for (int s = 0; s < n; ++s) {
+        double o = mlp_sample(w, x + (size_t)s * MI, xn, z1, a1, z2, a2);
+        double t = (y[s] - w[32]) / w[33];
+        double e = o - t;
+        loss += e * e;
+        double dout = 2.0 * e / n;
+        grad[OFF_B3] += dout;
+        for (int k = 0; k < MH; ++k) {
+            grad[OFF_W3 + k] += dout * a2[k];
+            d2[k] = dout * w[OFF_W3 + k] * gelu_d(z2[k]);
+        }
+        for (int j = 0; j < MH; ++j) {
+            grad[OFF_B2 + j] += d2[j];
+            for (int k = 0; k < MH; ++k) grad[OFF_W2 + j * MH + k] += d2[j] * a1[k];
+        }
+        for (int k = 0; k < MH; ++k) {
+            double s2 = 0.0;
+            for (int j = 0; j < MH; ++j) s2 += d2[j] * w[OFF_W2 + j * MH + k];
+            d1[k] = s2 * gelu_d(z1[k]);
+        }
+        for (int j = 0; j < MH; ++j) {
+            grad[OFF_B1 + j] += d1[j];
+            for (int i = 0; i < MI; ++i) grad[OFF_W1 + j * MI + i] += d1[j] * xn[i];
+        }
+    }
+    return loss / n;
+}

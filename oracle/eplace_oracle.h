@@ -167,6 +167,14 @@ double eo_gradient(const eo_design *d, const double *x, const double *y, double 
                    double gamma, double lambda, double *gx, double *gy, double *hpwl,
                    double *gwx, double *gwy, double *fx, double *fy, double *ex, double *ey);
 
+/* ---------------- step-length MLP (no reference counterpart) ---------------- */
+/* 16 -> 64 -> 64 -> 1, GELU (x/2 (1 + erf(x/sqrt 2))), blob layout as the
+ * product's include/eplace_mlp.h.  Forward returns log10 step (denormalised);
+ * grad returns the MSE loss on the normalised target and d loss / d blob. */
+int eo_mlp_nweights(void);
+void eo_mlp_forward(const double *w, const double *x, int n, double *out);
+double eo_mlp_grad(const double *w, const double *x, const double *y, int n, double *grad);
+
 /* Sets the OpenMP thread count used by the oracle (0 = leave default). */
 void eo_set_threads(int n);
 int eo_get_threads(void);

@@ -323,3 +323,46 @@ class Design:
         log = [dict(iter=l.iter, evals=l.evals, hpwl=l.hpwl, overflow=l.overflow,
                     lam=l.lambda_, gamma=l.gamma, alpha=l.alpha) for l in logs[:min(it, log_cap)]]
         return x, y, dict(iters=it, hpwl=fh.value, overflow=fo.value, evals=ev.value, log=log)
+
+
+# ---------------------------------------------------------------- step-length MLP
+
+MLP_IN, MLP_HID = 16, 64
+
+
+def mlp_nweights() -> int:
+    L = lib()
+    L.eo_mlp_nweights.restype = C.c_int
+    return int(L.eo_mlp_nweights())
+
+
+def mlp_forward(w, x):
+    L = lib()
+    L.eo_mlp_forward.argtypes = [_dp, _dp, C.c_int, _dp]
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, MLP_IN)
+    out = np.zeros(len(x))
+    L.eo_mlp_forward(w, x, len(x), out)
+    return out
+
+
+def mlp_grad(w, x, y):
+    L = lib()
+    L.eo_mlp_grad.argtypes = [_dp, _dp, _dp, C.c_int, _dp]
+    L.eo_mlp_grad.restype = C.c_double
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, MLP_IN)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    g = np.zeros(len(w))
+    loss = L.eo_mlp_grad(w, x, y, len(y), g)
+    return g, float(loss)
+
+
+def mlp_random_weights(rng, scale=0.3):
+    """A valid blob: normalisation header + random layer weights (tests)."""
+    n = mlp_nweights()
+    w = rng.standard_normal(n) * scale
+    w[0:16] = rng.standard_normal(16)
+    w[16:32] = rng.uniform(0.5, 2.0, 16)
+    w[32], w[33] = rng.standard_normal(), rng.uniform(0.5, 2.0)
+    return w

@@ -1,0 +1,148 @@
+"""Learned step length: device MLP kernels against the FP64 oracle, training,
+and the verified proposal inside the Nesterov loop (acceptance, fallback,
+and what the trained network gains over the plain search)."""
+import sys
+
+import numpy as np
+import pytest
+
+import aieplace_amd as A
+from aieplace_amd import api, learned
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def f32_blob(oracle, seed):
+    return oracle.mlp_random_weights(np.random.default_rng(seed)).astype(np.float32)
+
+
+def test_mlp_forward_and_grad_match_oracle(oracle):
+    say("case mlp forward/grad vs oracle")
+    rng = np.random.default_rng(3)
+    w = f32_blob(oracle, 3)
+    x = rng.standard_normal((300, 16)).astype(np.float32)
+    y = rng.standard_normal(300).astype(np.float32)
+    out = learned.forward(w, x)
+    oout = oracle.mlp_forward(w.astype(np.float64), x.astype(np.float64))
+    assert np.abs(out - oout).max() <= 1e-4 * max(1.0, np.abs(oout).max())
+    g, loss = learned.grad(w, x, y)
+    og, oloss = oracle.mlp_grad(w.astype(np.float64), x.astype(np.float64), y.astype(np.float64))
+    assert loss == pytest.approx(oloss, rel=1e-4)
+    err = np.abs(g - og).max() / np.abs(og).max()
+    say(f"  grad rel err (max-norm): {err:.2e}")
+    assert err <= 1e-4
+    assert np.all(g[:34] == 0)
+
+
+def test_mlp_training_reduces_loss():
+    say("case mlp training")
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((4000, 16)).astype(np.float32)
+    y = (np.tanh(x[:, 0] - 0.5 * x[:, 3]) + 0.3 * x[:, 7] ** 2).astype(np.float32)
+    w, curve = learned.train(x, y, epochs=30, batch=256)
+    say(f"  loss: {curve[0]:.4f} -> {curve[-1]:.4f}")
+    assert curve[-1] < 0.1 * curve[0]
+    pred = learned.forward(w, x)
+    assert np.mean((pred - y) ** 2) < 0.1 * np.var(y)
+
+
+def _instance(seed, n=20_000, bins=128, **kw):
+    nl = A.generate("stdcell", seed=seed, n_movable=n)
+    p = A.placer_params(A.CONFIGS["stdcell_200k"], bins_x=bins, bins_y=bins, **kw)
+    return nl, p
+
+
+@pytest.fixture(scope="module")
+def small_model():
+    """Trained on records of 4 plain runs (seeds 11-14) by the device kernels."""
+    feats, targs = [], []
+    for s in (11, 12, 13, 14):
+        nl, p = _instance(s, use_graph=1)
+        with A.Placer(nl, p) as pl:
+            pl.run()
+            f, t = pl.take_step_records()
+        feats.append(f)
+        targs.append(t)
+    X, Y = np.concatenate(feats), np.concatenate(targs)
+    w, curve = learned.train(X, Y, epochs=25, batch=256)
+    return w, curve, len(Y)
+
+
+def test_records_are_collected():
+    nl, p = _instance(21, use_graph=1)
+    with A.Placer(nl, p) as pl:
+        res = pl.run()
+        f, t = pl.take_step_records()
+        assert len(t) == res["iterations"] and f.shape == (len(t), 16)
+        assert np.all(np.isfinite(f)) and np.all(np.isfinite(t))
+        f2, _ = pl.take_step_records()
+        assert len(f2) == 0  # taken records are cleared
+
+
+def _ablation(w, seed, n, bins, precision=api.EP_PREC_BF16X3):
+    nl, p = _instance(seed, n=n, bins=bins, use_graph=1, precision=precision)
+    with A.Placer(nl, p) as pl:
+        plain = pl.run()
+    p.learned_step = 1
+    with A.Placer(nl, p) as pl:
+        pl.set_step_model(w)
+        lrn = pl.run()
+    rate = lrn["learned_accepted"] / max(1, lrn["learned_proposals"])
+    say(f"  same form without model: {plain['iterations']} it {plain['evals']} evals "
+        f"hpwl {plain['hpwl']:.1f} {plain['seconds']:.3f}s; with model: {lrn['iterations']} it "
+        f"{lrn['evals']} evals hpwl {lrn['hpwl']:.1f} {lrn['seconds']:.3f}s; acceptance {rate:.3f} "
+        f"({lrn['evals'] / lrn['iterations']:.3f} vs {plain['evals'] / plain['iterations']:.3f} evals/it)")
+    assert lrn["overflow"] <= 0.10
+    assert lrn["learned_proposals"] == lrn["iterations"]
+    return plain, lrn, rate
+
+
+def test_learned_step_small_model(small_model):
+    """A model trained on 4 runs only: every proposal is checked; rejected ones
+    fall back to the plain search, and the run still converges."""
+    w, curve, nrec = small_model
+    say(f"case learned step (4-run model, {nrec} records, loss {curve[0]:.3f} -> {curve[-1]:.3f})")
+    plain, lrn, rate = _ablation(w, 31, 20_000, 128)
+    assert lrn["hpwl"] <= plain["hpwl"] * 1.005
+
+
+def test_learned_step_ablation_committed_model():
+    """The committed model (200 seeded runs, 50k-500k cells, trained by the
+    device kernels): acceptance rate, evaluations per iteration and HPWL against
+    the same form without the model, on held-out instances."""
+    import os
+    if not os.path.exists(learned.DEFAULT_WEIGHTS):
+        pytest.skip("no committed step model")
+    w = learned.load()
+    for seed, n, bins in ((31, 20_000, 128), (32, 200_000, 512)):
+        say(f"case ablation seed={seed} n={n} bins={bins}")
+        plain, lrn, rate = _ablation(w, seed, n, bins)
+        assert rate > 0.5
+        assert lrn["hpwl"] <= plain["hpwl"] * 1.005
+
+
+def test_corrupted_model_falls_back():
+    """A model proposing absurd steps is rejected by the check every time and
+    the plain search still converges."""
+    say("case corrupted step model")
+    w = np.zeros(api.MLP_NWEIGHTS, np.float32)
+    w[16:32] = 1.0
+    w[32], w[33] = 6.0, 1.0  # always proposes 10^6
+    nl, p = _instance(41, use_graph=1, learned_step=1)
+    with A.Placer(nl, p) as pl:
+        pl.set_step_model(w)
+        res = pl.run()
+    say(f"  proposals {res['learned_proposals']} accepted {res['learned_accepted']} "
+        f"overflow {res['overflow']:.4f}")
+    assert res["overflow"] <= 0.10
+    assert res["learned_accepted"] <= 0.02 * res["learned_proposals"]
+    with pytest.raises(api.EplaceError):
+        bad = w.copy()
+        bad[100] = np.nan
+        with A.Placer(nl, p) as pl:
+            pl.set_step_model(bad)
