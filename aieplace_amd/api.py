@@ -83,7 +83,7 @@ class EpDesign(C.Structure):
 EXPORTS = [
     "ep_default_params", "ep_create", "ep_plan_fillers", "ep_destroy", "ep_last_error", "ep_get_sizes",
     "ep_get_cells", "ep_get_nets", "ep_set_positions", "ep_get_positions", "ep_wa_grad",
-    "ep_get_wa_grad", "ep_density", "ep_get_density", "ep_field", "ep_get_field", "ep_gradient",
+    "ep_get_wa_grad", "ep_density", "ep_get_density", "ep_field", "ep_get_field", "ep_field_error", "ep_gradient",
     "ep_cell_bins", "ep_hpwl", "ep_overflow", "ep_init_optimizer", "ep_step", "ep_run",
     "ep_set_precision", "ep_profile", "ep_take_step_records", "ep_set_step_model", "ep_wa_packet",
     "ep_gen_default_params", "ep_gen_create", "ep_gen_free",
@@ -131,6 +131,7 @@ def _load():
     L.ep_get_density.argtypes = [vp, _fp, _fp, _fp]
     L.ep_field.argtypes = [vp, C.c_int32]
     L.ep_get_field.argtypes = [vp, _fp, _fp]
+    L.ep_field_error.argtypes = [vp, C.c_int32, _dp, _dp]
     L.ep_gradient.argtypes = [vp, C.c_float, C.c_float, _fp, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
     L.ep_cell_bins.argtypes = [vp, C.c_int32, _ip]
     L.ep_hpwl.argtypes = [vp, _dp]

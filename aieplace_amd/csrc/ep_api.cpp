@@ -200,9 +200,8 @@ extern "C" void ep_default_params(ep_params *p)
 
 static ep_status check_precision(int32_t precision, const char *who)
 {
-    if (precision == EP_PREC_FP32 || precision == EP_PREC_BF16X3) return EP_OK;
-    if (precision == EP_PREC_FP16X3)
-        return fail(EP_ENOTSUP, "%s: split-FP16 transforms not available in this build", who);
+    if (precision == EP_PREC_FP32 || precision == EP_PREC_BF16X3 || precision == EP_PREC_FP16X3)
+        return EP_OK;
     return fail(EP_EINVAL, "%s: unknown precision %d", who, precision);
 }
 
@@ -472,7 +471,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     c->cell_blocks = (int)std::min<int64_t>(kCellGridCap, ((int64_t)c->n_place + kBlock - 1) / kBlock);
     c->part_wa_off = 0;
     c->part_ovf_off = 2 * c->wa_part_blocks;
-    c->part_cell_off = c->part_ovf_off + c->ovf_blocks;
+    c->part_cell_off = c->part_ovf_off + 4 * c->ovf_blocks;
     c->part_total = c->part_cell_off + 7 * c->cell_blocks;
     std::vector<float> fixmap;
     fixed_map(*c, wh, pos, fixmap);
@@ -675,6 +674,16 @@ extern "C" ep_status ep_field(ep_ctx *c, int32_t precision)
     ep_status st = check_launch();
     if (st != EP_OK) return st;
     return sync(*c);
+}
+
+extern "C" ep_status ep_field_error(ep_ctx *c, int32_t precision, double *err_ex, double *err_ey)
+{
+    if (!c) return fail(EP_EINVAL, "ep_field_error: null context");
+    ep_status st = check_precision(precision, "ep_field_error");
+    if (st != EP_OK) return st;
+    launch_field(*c, precision, c->stream);
+    if ((st = check_launch()) != EP_OK) return st;
+    return field_error(*c, err_ex, err_ey);
 }
 
 extern "C" ep_status ep_get_field(ep_ctx *c, float *ex, float *ey)
@@ -1137,6 +1146,32 @@ extern "C" ep_status ep_set_step_model(ep_ctx *c, const float *w, int32_t n)
 
 extern "C" ep_status ep_wa_packet(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out)
 {
-    (void)in; (void)n_in; (void)gamma; (void)out; (void)n_out;
-    return fail(EP_ENOTSUP, "ep_wa_packet: not built yet");
+    if (!in || !out || n_in < 4) return fail(EP_EINVAL, "ep_wa_packet: bad argument");
+    if (!(gamma > 0.f)) return fail(EP_EINVAL, "ep_wa_packet: gamma must be > 0");
+    const float fs = in[0], fn = in[1];
+    if (!(fs >= 1.f) || fs != std::floor(fs) || !(fn >= 0.f) || fn != std::floor(fn) || fs > 65536.f ||
+        fn > 1e8f)
+        return fail(EP_EINVAL, "ep_wa_packet: bad control words [%g, %g]", fs, fn);
+    const int net_size = (int)fs, n_nets = (int)fn;
+    const int groups = (n_nets + 7) / 8;
+    const int64_t n_data = (int64_t)groups * net_size * 8;
+    if (n_in < 4 + n_data) return fail(EP_EINVAL, "ep_wa_packet: packet holds %d values, needs %lld", n_in - 4, (long long)n_data);
+    if (n_out < n_data) return fail(EP_EINVAL, "ep_wa_packet: output holds %d values, needs %lld", n_out, (long long)n_data);
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) return fail(EP_EDEVICE, "ep_wa_packet: no HIP device");
+    if (n_data == 0) return EP_OK;
+    float *d_in = nullptr, *d_out = nullptr;
+    ep_status st = dalloc(&d_in, (size_t)n_data);
+    if (st == EP_OK) st = dalloc(&d_out, (size_t)n_data);
+    if (st == EP_OK && hipMemcpy(d_in, in + 4, sizeof(float) * n_data, hipMemcpyHostToDevice) != hipSuccess)
+        st = fail(EP_EDEVICE, "ep_wa_packet: upload failed");
+    if (st == EP_OK) {
+        launch_wa_packet(d_in, net_size, n_nets, groups, 1.0f / gamma, d_out, nullptr);
+        if ((st = check_launch()) == EP_OK &&
+            hipMemcpy(out, d_out, sizeof(float) * n_data, hipMemcpyDeviceToHost) != hipSuccess)
+            st = fail(EP_EDEVICE, "ep_wa_packet: download failed");
+    }
+    if (d_in) (void)hipFree(d_in);
+    if (d_out) (void)hipFree(d_out);
+    return st;
 }

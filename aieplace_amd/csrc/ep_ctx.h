@@ -90,8 +90,12 @@ void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, flo
                   float2 *u_new, hipStream_t s);
 void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s);
 void launch_mlp_propose(Ctx &c, hipStream_t s);
+void launch_wa_packet(const float *data, int net_size, int n_nets, int groups, float ginv,
+                      float *out, hipStream_t s);
 void launch_cell_bins(Ctx &c, const float2 *pos, int stretched, int32_t *out, hipStream_t s);
 ep_status field_init(Ctx &c);   // builds transform matrices
+// FP64 device solve of the current map; max-norm relative error of ex/ey against it.
+ep_status field_error(Ctx &c, double *err_ex, double *err_ey);
 void field_free(Ctx &c);
 
 struct Ctx {
@@ -127,6 +131,8 @@ struct Ctx {
     float *ex = nullptr, *ey = nullptr;
     float *tA = nullptr, *tB = nullptr;         // transform scratch, 2*Mx*My each
     void *split_mats = nullptr;                 // host handle: FP32 + split transform matrices
+    uint32_t *field_max = nullptr;              // [4] running max |operand| for split FP16
+    float *map_mean = nullptr;                  // mean of the field map (removed before the solve)
     int32_t field_precision_built = -1;
 
     // reductions

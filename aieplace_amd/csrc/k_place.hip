@@ -359,16 +359,46 @@ __global__ __launch_bounds__(kBlock) void k_density(const int32_t *__restrict__ 
 }
 
 // Overflow numerator per block: sum_b max(0, mov_b + rho_t*fix_b - rho_t*A_b).
+// Overflow numerator plus the field map's sum, max and min per block (the
+// transforms subtract the map mean; split FP16 scales by the centred range).
+// part layout per block: [ovf, sum, max, -min].
 __global__ __launch_bounds__(kBlock) void k_overflow(const float *__restrict__ map_mov,
-                                                     const float *__restrict__ map_fix, float rho_t,
+                                                     const float *__restrict__ map_fix,
+                                                     const float *__restrict__ map_field, float rho_t,
                                                      float cap, int nb, double *__restrict__ part)
 {
-    double acc[1] = {0.0};
+    double acc[4] = {0.0, 0.0, -1e300, -1e300};
     for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock) {
-        float v = map_mov[b] + rho_t * map_fix[b] - cap;
+        const float v = map_mov[b] + rho_t * map_fix[b] - cap;
         if (v > 0.0f) acc[0] += (double)v;
+        const double f = map_field[b];
+        acc[1] += f;
+        acc[2] = fmax(acc[2], f);
+        acc[3] = fmax(acc[3], -f);
     }
-    block_reduce<1, 0>(acc, part + blockIdx.x);
+    block_reduce<4, 2>(acc, part + 4 * blockIdx.x);
+}
+
+// Map mean and centred range for the field solve: mean -> stats[0] (float),
+// max |map - mean| -> field_max0 (float bits).  One block.
+__global__ __launch_bounds__(kBlock) void k_map_stats(const double *__restrict__ part, int blocks,
+                                                      int nb, float *__restrict__ mean_out,
+                                                      uint32_t *__restrict__ field_max0)
+{
+    double v[3] = {0.0, -1e300, -1e300};
+    for (int b = threadIdx.x; b < blocks; b += kBlock) {
+        v[0] += part[4 * b + 1];
+        v[1] = fmax(v[1], part[4 * b + 2]);
+        v[2] = fmax(v[2], part[4 * b + 3]);
+    }
+    __shared__ double out[3];
+    block_reduce<3, 2>(v, out);
+    if (threadIdx.x == 0) {
+        const double mean = out[0] / nb;
+        *mean_out = (float)mean;
+        const double r = fmax(out[1] - mean, -out[2] + mean);
+        *field_max0 = __float_as_uint((float)fmax(r, 0.0));
+    }
 }
 
 // ------------------------------------------------------------------ WA gradient
@@ -533,6 +563,44 @@ __global__ __launch_bounds__(kBlock) void k_wa(BucketTable bt, const int32_t *__
     block_reduce<2, 0>(v, part + 2 * blockIdx.x);
 }
 
+// ------------------------------------------------------------------ reference packet format
+
+// WA partials straight from the AIE packet layout (DataBase.cpp:217-273,
+// compute_abc.cpp:27-57): groups of 8 lanes (one net per lane), `net_size`
+// vectors of 8 values per group, pin 0 = net maximum.  Output in the same
+// (group, pin, lane) order (compute_partials.cpp:11-63).  Lanes past n_nets
+// are zero padding and get 0.  One thread per lane.
+__global__ void k_wa_packet(const float *__restrict__ data, int net_size, int n_nets, int groups,
+                            float ginv, float *__restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= groups * 8) return;
+    const int g = t >> 3, lane = t & 7;
+    const float *v = data + (size_t)g * net_size * 8 + lane;
+    float *o = out + (size_t)g * net_size * 8 + lane;
+    if (t >= n_nets) {
+        for (int p = 0; p < net_size; ++p) o[p * 8] = 0.0f;
+        return;
+    }
+    float xmx = -FLT_MAX, xmn = FLT_MAX;
+    for (int p = 0; p < net_size; ++p) {
+        xmx = fmaxf(xmx, v[p * 8]);
+        xmn = fminf(xmn, v[p * 8]);
+    }
+    float bp = 0.f, bm = 0.f, cp = 0.f, cm = 0.f;
+    for (int p = 0; p < net_size; ++p) {
+        const float dp = v[p * 8] - xmx, dm = v[p * 8] - xmn;
+        const float ap = __expf(dp * ginv), am = __expf(-dm * ginv);
+        bp += ap; bm += am; cp += dp * ap; cm += dm * am;
+    }
+    const float mp = cp / bp, mm = cm / bm;
+    for (int p = 0; p < net_size; ++p) {
+        const float dp = v[p * 8] - xmx, dm = v[p * 8] - xmn;
+        const float ap = __expf(dp * ginv), am = __expf(-dm * ginv);
+        o[p * 8] = ap / bp * (1.f + (dp - mp) * ginv) - am / bm * (1.f - (dm - mm) * ginv);
+    }
+}
+
 // ------------------------------------------------------------------ per-cell gather
 
 // g = (gW - lambda * F) / max(1, pins + lambda * area); F = r * sum ov * E over
@@ -617,7 +685,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
         v[0] += part[wa_off + 2 * b];
         v[1] += part[wa_off + 2 * b + 1];
     }
-    for (int b = threadIdx.x; b < ovf_blocks; b += kFinalizeThreads) v[2] += part[ovf_off + b];
+    for (int b = threadIdx.x; b < ovf_blocks; b += kFinalizeThreads) v[2] += part[ovf_off + 4 * b];
     for (int b = threadIdx.x; b < cell_blocks; b += kFinalizeThreads) {
         const double *q = part + cell_off + 7 * b;
         v[3] += q[0]; v[4] += q[1]; v[5] += q[2]; v[6] += q[3]; v[7] += q[4]; v[8] += q[5];
@@ -748,7 +816,9 @@ void launch_overflow(Ctx &c, hipStream_t s)
     const int nb = c.Mx * c.My;
     const float cap = c.prm.target_density * c.bw * c.bh;
     hipLaunchKernelGGL(k_overflow, dim3(c.ovf_blocks), dim3(kBlock), 0, s, c.map_mov, c.map_fix,
-                       c.prm.target_density, cap, nb, c.part + c.part_ovf_off);
+                       c.map_field, c.prm.target_density, cap, nb, c.part + c.part_ovf_off);
+    hipLaunchKernelGGL(k_map_stats, dim3(1), dim3(kBlock), 0, s, c.part + c.part_ovf_off,
+                       c.ovf_blocks, nb, c.map_mean, c.field_max);
 }
 
 void launch_wa(Ctx &c, const float2 *pos, hipStream_t s)
@@ -786,6 +856,14 @@ void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipSt
 {
     hipLaunchKernelGGL(k_probe, dim3(grid_for(c.n_place, kBlock, 8192)), dim3(kBlock), 0, s, v, g,
                        c.wh, c.n_place, geom_args(c), c.d_scal, v_new);
+}
+
+void launch_wa_packet(const float *data, int net_size, int n_nets, int groups, float ginv,
+                      float *out, hipStream_t s)
+{
+    const int threads = groups * 8;
+    hipLaunchKernelGGL(k_wa_packet, dim3((threads + 255) / 256), dim3(256), 0, s, data, net_size,
+                       n_nets, groups, ginv, out);
 }
 
 void launch_cell_bins(Ctx &c, const float2 *pos, int stretched, int32_t *out, hipStream_t s)

@@ -115,6 +115,12 @@ class Placer:
         api.check(api.lib.ep_get_field(self._ctx, api.fptr(ex), api.fptr(ey)))
         return ex.reshape(self.bins), ey.reshape(self.bins)
 
+    def field_error(self, precision: int):
+        """(err_ex, err_ey): max-norm error of the field in `precision` vs an FP64 device solve."""
+        ex, ey = C.c_double(), C.c_double()
+        api.check(api.lib.ep_field_error(self._ctx, int(precision), C.byref(ex), C.byref(ey)))
+        return ex.value, ey.value
+
     def gradient(self, gamma: float, lam: float):
         n = self.n_place
         out = {k: np.zeros(n, np.float32) for k in ("gx", "gy", "gwx", "gwy", "fx", "fy")}

@@ -65,6 +65,29 @@ def run_form(A, nl, cfg, form, steps, warmup):
     return pl, runs
 
 
+def precision_report(A, nl, cfg, every=50):
+    """Untimed: the split-BF16 form run step by step; every `every` iterations the
+    field of the current map is solved in each precision and compared with an
+    FP64 device solve (max-norm relative error)."""
+    from aieplace_amd import api
+    p = A.placer_params(cfg, precision=api.EP_PREC_BF16X3, use_graph=1)
+    rep = {"every": every, "iteration": [], "fp32": [], "bf16x3": [], "fp16x3": []}
+    with A.Placer(nl, p) as pl:
+        pl.initializePlacement()
+        while True:
+            r = pl.nudgeAllNodes()
+            if r["iteration"] % every == 0 or r["overflow"] <= p.stop_overflow:
+                rep["iteration"].append(r["iteration"])
+                for name, prec in (("fp32", api.EP_PREC_FP32), ("bf16x3", api.EP_PREC_BF16X3),
+                                   ("fp16x3", api.EP_PREC_FP16X3)):
+                    rep[name].append(float("%.3g" % max(pl.field_error(prec))))
+            if r["overflow"] <= p.stop_overflow or r["iteration"] >= p.max_iters - 1:
+                break
+    for name in ("fp32", "bf16x3", "fp16x3"):
+        rep[name + "_max"] = max(rep[name])
+    return rep
+
+
 def profile(pl, reps=20):
     from aieplace_amd import api
     stats = (api.EpKernelStat * 16)()
@@ -143,6 +166,7 @@ def main():
     ap.add_argument("--workload", default="stdcell_2m")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision-report", action="store_true")
     ap.add_argument("--forms", default="plain,mixed,mixed_learned")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -199,6 +223,8 @@ def main():
         out["speedup_vs_plain"] = round(plain["seconds"] / m["seconds"], 3)
         out["hpwl_gap_pct"] = round(100.0 * (m["hpwl"] - plain["hpwl"]) / plain["hpwl"], 3)
 
+    if not args.no_precision_report:
+        out["field_error_vs_fp64"] = precision_report(A, nl, cfg)
     stats = profile(placers["plain"])
     dom, roof = roofline_of(stats)
     out["roofline"] = dict(kernel=dom["name"], **roof)

@@ -151,6 +151,10 @@ ep_status ep_get_density(ep_ctx *ctx, float *field_map, float *mov_map, float *f
 /* Poisson field of the current field map with the given precision. */
 ep_status ep_field(ep_ctx *ctx, int32_t precision);
 ep_status ep_get_field(ep_ctx *ctx, float *ex, float *ey);
+/* Field error report: solves the current field map with `precision` and with an
+ * FP64 device reference solve; returns max|E - E64| / max|E64| for x and y.
+ * Leaves the `precision` field in place (as ep_field). */
+ep_status ep_field_error(ep_ctx *ctx, int32_t precision, double *err_ex, double *err_ey);
 /* Full preconditioned gradient (density + field + WA + gather) with explicit
  * gamma/lambda.  Outputs (nullable) are n_mov+n_fill: g = (gW - lambda F)/P,
  * and the components gW and F. */

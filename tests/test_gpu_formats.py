@@ -1,0 +1,118 @@
+"""Reference wire format on the device (AIE packet layout) and the split-FP16
+transform form with its FP64 error report."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import aieplace_amd as A
+from aieplace_amd import api
+from support import oracle_design
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures", "partials")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rd(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return np.array([float(l) for l in f if l.strip()], np.float32)
+
+
+def packet(inp, gamma):
+    import ctypes as C
+    out = np.zeros(len(inp) - 4, np.float32)
+    inp = np.ascontiguousarray(inp, np.float32)
+    rc = api.lib.ep_wa_packet(api.fptr(inp), len(inp), C.c_float(gamma), api.fptr(out), len(out))
+    return rc, out
+
+
+@pytest.mark.parametrize("bench", range(4))
+def test_wa_packet_golden_replay(oracle, bench):
+    """x_in{b}.dat -> partials{b}.dat (gamma 4) with the reference comparer's
+    tolerance (test/compare_output.py:13,56-57: rel 0.01 or abs 1e-2), and
+    against the FP64 oracle with true exp."""
+    say(f"case wa packet golden {bench}")
+    xin = rd(f"x_in{bench}.dat")
+    rc, out = packet(xin, 4.0)
+    assert rc == api.EP_OK
+    gold = rd(f"partials{bench}.dat")
+    ok = np.isclose(out, gold, rtol=0.01) | (np.abs(out - gold) < 1e-2)
+    assert ok.all()
+    xs = xin[4:].astype(np.float64).reshape(2, 8)
+    ref = np.stack([oracle.wa_partials(xs[:, l], 4.0) for l in range(8)], axis=1).ravel()
+    np.testing.assert_allclose(out, ref, rtol=1e-4, atol=1e-5)
+
+
+def test_wa_packet_wider_nets_and_padding(oracle):
+    rng = np.random.default_rng(2)
+    size, n = 7, 13  # two groups, 3 padded lanes
+    groups = 2
+    vals = rng.uniform(0, 5000, (groups, size, 8)).astype(np.float32)
+    vals[1, :, 5:] = 0.0
+    inp = np.concatenate([[size, n, 0, 0], vals.ravel()]).astype(np.float32)
+    rc, out = packet(inp, 3.0)
+    assert rc == api.EP_OK
+    out = out.reshape(groups, size, 8)
+    for g in range(groups):
+        for l in range(8):
+            if g * 8 + l < n:
+                np.testing.assert_allclose(out[g, :, l], oracle.wa_partials(vals[g, :, l], 3.0),
+                                           rtol=1e-4, atol=1e-5)
+            else:
+                assert np.all(out[g, :, l] == 0)
+    bad = inp.copy()
+    bad[0] = 2.5
+    assert packet(bad, 3.0)[0] == api.EP_EINVAL
+    assert packet(inp[:20], 3.0)[0] == api.EP_EINVAL
+
+
+@pytest.mark.parametrize("bins", [32, 128, 512, 1024])
+def test_field_fp16x3_and_error_report(oracle, bins):
+    """Split-FP16 products with power-of-two operand scaling; the FP64 device
+    reference solve reports the error of each form (checked against the oracle)."""
+    say(f"case field fp16x3 bins={bins}")
+    n = {32: 5_000, 128: 20_000, 512: 100_000, 1024: 200_000}[bins]
+    nl = A.generate("stdcell", seed=1, n_movable=n)
+    p = A.placer_params(A.CONFIGS["stdcell_200k"], bins_x=bins, bins_y=bins)
+    d, _, _ = oracle_design(oracle, nl, p)
+    with A.Placer(nl, p) as pl:
+        rng = np.random.default_rng(2)
+        xl, yl, xh, yh = nl.die
+        x = (xl + rng.random(d.n_place) * (xh - xl) * 0.7).astype(np.float32)
+        y = (yl + rng.random(d.n_place) * (yh - yl) * 0.7).astype(np.float32)
+        pl.set_positions(x, y)
+        fmap, _, _, _ = pl.computeOverlaps()
+        bw = (xh - xl) / bins
+        _, oex, oey = oracle.field(fmap.astype(np.float64) / (bw * bw), wy_scale=1.0)
+        errs = {}
+        for name, prec in (("fp32", api.EP_PREC_FP32), ("bf16x3", api.EP_PREC_BF16X3),
+                           ("fp16x3", api.EP_PREC_FP16X3)):
+            rex, rey = pl.field_error(prec)
+            ex, ey = pl.computeElectricFields(prec)
+            oerr = max(np.abs(ex - oex).max() / np.abs(oex).max(), np.abs(ey - oey).max() / np.abs(oey).max())
+            errs[name] = (max(rex, rey), oerr)
+            # the device FP64 reference agrees with the oracle's FP64 solve
+            assert max(rex, rey) == pytest.approx(oerr, rel=0.05, abs=2e-7)
+        say("  " + "  ".join(f"{k}: device-reported {v[0]:.2e} (oracle {v[1]:.2e})" for k, v in errs.items()))
+        assert errs["fp32"][1] <= 1e-5
+        assert errs["fp16x3"][1] <= 2e-4 and errs["bf16x3"][1] <= 2e-4
+
+
+def test_placement_fp16x3():
+    nl = A.generate("stdcell", seed=1, n_movable=20_000)
+    p = A.placer_params(A.CONFIGS["stdcell_200k"], bins_x=128, bins_y=128)
+    res = {}
+    for prec in (api.EP_PREC_FP32, api.EP_PREC_FP16X3):
+        p.precision = prec
+        with A.Placer(nl, p) as pl:
+            res[prec] = pl.run()
+    a, b = res[api.EP_PREC_FP32], res[api.EP_PREC_FP16X3]
+    say(f"case placement fp16x3: fp32 hpwl {a['hpwl']:.1f}, fp16x3 hpwl {b['hpwl']:.1f}")
+    assert b["overflow"] <= 0.10
+    assert b["hpwl"] == pytest.approx(a["hpwl"], rel=5e-3)
