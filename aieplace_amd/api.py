@@ -88,6 +88,9 @@ EXPORTS = [
     "ep_set_precision", "ep_profile", "ep_take_step_records", "ep_set_step_model", "ep_wa_packet",
     "ep_gen_default_params", "ep_gen_create", "ep_gen_free",
     "ep_mlp_default_train_params", "ep_mlp_train", "ep_mlp_forward", "ep_mlp_grad",
+    "ep_batch_create", "ep_batch_destroy", "ep_batch_reset", "ep_batch_init", "ep_batch_run",
+    "ep_batch_results", "ep_batch_sizes", "ep_batch_positions", "ep_batch_gradient",
+    "ep_batch_state",
 ]
 
 MLP_IN, MLP_HID = 16, 64
@@ -98,6 +101,13 @@ class EpMlpTrainParams(C.Structure):
     _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("seed", C.c_uint64)]
+
+
+class EpBatchRunInfo(C.Structure):
+    _fields_ = [("instances", C.c_int32), ("finished", C.c_int32),
+                ("instance_iterations", C.c_int64), ("evals", C.c_int64),
+                ("launches", C.c_int32), ("seconds", C.c_double),
+                ("density_cells_per_thread", C.c_int32)]
 
 
 class EplaceError(RuntimeError):
@@ -154,6 +164,17 @@ def _load():
     L.ep_mlp_train.argtypes = [_fp, _fp, C.c_int32, C.c_void_p, _fp, _fp]
     L.ep_mlp_forward.argtypes = [_fp, _fp, C.c_int32, _fp]
     L.ep_mlp_grad.argtypes = [_fp, _fp, _fp, C.c_int32, _fp, _fp]
+    L.ep_batch_create.argtypes = [C.POINTER(C.POINTER(EpNetlist)), _ip, C.c_int32,
+                                  C.POINTER(EpParams), C.POINTER(vp)]
+    L.ep_batch_destroy.argtypes = [vp]
+    L.ep_batch_reset.argtypes = [vp]
+    L.ep_batch_init.argtypes = [vp]
+    L.ep_batch_run.argtypes = [vp, C.c_int32, C.POINTER(EpBatchRunInfo)]
+    L.ep_batch_results.argtypes = [vp, _ip, C.POINTER(C.c_int64), _dp, _dp]
+    L.ep_batch_sizes.argtypes = [vp, C.c_int32, _ip, _ip, _ip]
+    L.ep_batch_positions.argtypes = [vp, C.c_int32, _fp, _fp]
+    L.ep_batch_gradient.argtypes = [vp, C.c_int32, _fp, _fp]
+    L.ep_batch_state.argtypes = [vp, C.c_int32, _dp, _dp, _dp]
     return L
 
 

@@ -17,7 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libeplace.so")
 RUNNER = os.path.join(HERE, "eplace_run")
 
-SOURCES = ["k_place.hip", "k_field.hip", "k_mlp.hip", "ep_api.cpp", "ep_mlp.cpp", "generator.cpp"]
+SOURCES = ["k_place.hip", "k_field.hip", "k_mlp.hip", "k_batch.hip", "ep_api.cpp", "batch.cpp",
+           "generator.cpp"]
 ARCH = os.environ.get("EPLACE_OFFLOAD_ARCH", "gfx950")
 
 

@@ -16,6 +16,7 @@
 
 #include "../../include/eplace_mlp.h"
 #include "ep_ctx.h"
+#include "ep_layout.h"
 
 namespace ep {
 
@@ -70,38 +71,7 @@ int lanes_for_degree(int d)
     return g;
 }
 
-// Exact overlap of fixed cells with the bins (Bin.h:78-121), host FP64, once.
-void fixed_map(const Ctx &c, const std::vector<float2> &wh, const std::vector<float2> &pos,
-               std::vector<float> &out)
-{
-    std::vector<double> m((size_t)c.Mx * c.My, 0.0);
-    const double bw = c.bw, bh = c.bh;
-    for (int k = c.n_place; k < c.n_all; ++k) {
-        const double x0 = pos[k].x, x1 = x0 + wh[k].x, y0 = pos[k].y, y1 = y0 + wh[k].y;
-        int i0 = std::max(0, (int)std::floor((x0 - c.xl) / bw));
-        int i1 = std::min(c.Mx - 1, (int)std::floor((x1 - c.xl) / bw));
-        int j0 = std::max(0, (int)std::floor((y0 - c.yl) / bh));
-        int j1 = std::min(c.My - 1, (int)std::floor((y1 - c.yl) / bh));
-        for (int i = i0; i <= i1; ++i) {
-            const double ox = std::min(c.xl + (i + 1) * bw, x1) - std::max(c.xl + i * bw, x0);
-            if (ox <= 0) continue;
-            for (int j = j0; j <= j1; ++j) {
-                const double oy = std::min(c.yl + (j + 1) * bh, y1) - std::max(c.yl + j * bh, y0);
-                if (oy <= 0) continue;
-                m[(size_t)i * c.My + j] += ox * oy;
-            }
-        }
-    }
-    out.resize(m.size());
-    for (size_t b = 0; b < m.size(); ++b) out[b] = (float)m[b];
-}
-
-double gamma_of(const Ctx &c, double tau)
-{
-    double g = (double)c.prm.gamma_bins * c.bw;
-    if (c.prm.gamma_schedule) g *= std::pow(10.0, (tau - 0.1) * 20.0 / 9.0 - 1.0);
-    return g;
-}
+double gamma_of(const Ctx &c, double tau) { return gamma_at(c.prm, c.bw, tau); }
 
 ep_status check_launch()
 {
@@ -205,7 +175,7 @@ static ep_status check_precision(int32_t precision, const char *who)
     return fail(EP_EINVAL, "%s: unknown precision %d", who, precision);
 }
 
-static ep_status validate(const ep_netlist *nl, const ep_params *p)
+ep_status ep::validate_netlist(const ep_netlist *nl, const ep_params *p)
 {
     if (!nl || !p) return fail(EP_EINVAL, "ep_create: null argument");
     if (nl->n_mov < 1 || nl->n_fix < 0) return fail(EP_EINVAL, "ep_create: need n_mov >= 1, n_fix >= 0");
@@ -270,7 +240,7 @@ static void destroy_ctx(ep_ctx *c)
 extern "C" ep_status ep_plan_fillers(const ep_netlist *nl, const ep_params *p, int32_t *n_fill,
                                      float *fw_out, float *fh_out, float *x, float *y, int32_t cap)
 {
-    ep_status st = validate(nl, p);
+    ep_status st = validate_netlist(nl, p);
     if (st != EP_OK) return st;
     double mov_area = 0, fix_area = 0, wsum = 0, hsum = 0;
     for (int k = 0; k < nl->n_mov; ++k) {
@@ -322,73 +292,92 @@ extern "C" ep_status ep_plan_fillers(const ep_netlist *nl, const ep_params *p, i
     return EP_OK;
 }
 
-extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx **out)
+// Exact overlap of fixed cells with the bins (Bin.h:78-121), host FP64, once.
+static void fixed_map(const HostLayout &L, std::vector<float> &out)
 {
-    if (!out) return fail(EP_EINVAL, "ep_create: null output");
-    *out = nullptr;
-    ep_status st = validate(nl, p);
-    if (st != EP_OK) return st;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-        return fail(EP_EDEVICE, "ep_create: no HIP device available");
+    std::vector<double> m((size_t)L.Mx * L.My, 0.0);
+    const double bw = L.bw, bh = L.bh;
+    for (int k = L.n_place; k < L.n_all; ++k) {
+        const double x0 = L.pos[k].x, x1 = x0 + L.wh[k].x, y0 = L.pos[k].y, y1 = y0 + L.wh[k].y;
+        int i0 = std::max(0, (int)std::floor((x0 - L.xl) / bw));
+        int i1 = std::min(L.Mx - 1, (int)std::floor((x1 - L.xl) / bw));
+        int j0 = std::max(0, (int)std::floor((y0 - L.yl) / bh));
+        int j1 = std::min(L.My - 1, (int)std::floor((y1 - L.yl) / bh));
+        for (int i = i0; i <= i1; ++i) {
+            const double ox = std::min(L.xl + (i + 1) * bw, x1) - std::max(L.xl + i * bw, x0);
+            if (ox <= 0) continue;
+            for (int j = j0; j <= j1; ++j) {
+                const double oy = std::min(L.yl + (j + 1) * bh, y1) - std::max(L.yl + j * bh, y0);
+                if (oy <= 0) continue;
+                m[(size_t)i * L.My + j] += ox * oy;
+            }
+        }
+    }
+    out.resize(m.size());
+    for (size_t b = 0; b < m.size(); ++b) out[b] = (float)m[b];
+}
 
-    ep_ctx *c = new (std::nothrow) ep_ctx();
-    if (!c) return fail(EP_ENOMEM, "ep_create: host allocation failed");
-    c->prm = *p;
-    (void)hipGetDevice(&c->device);
-    c->Mx = p->bins_x;
-    c->My = p->bins_y;
-    c->xl = nl->xl; c->yl = nl->yl; c->xh = nl->xh; c->yh = nl->yh;
-    c->bw = (float)(((double)nl->xh - nl->xl) / c->Mx);
-    c->bh = (float)(((double)nl->yh - nl->yl) / c->My);
-    c->sqrt2_bw = (float)(M_SQRT2 * (double)c->bw);
-    c->sqrt2_bh = (float)(M_SQRT2 * (double)c->bh);
-    c->n_mov = nl->n_mov;
-    c->n_fix = nl->n_fix;
+double ep::gamma_at(const ep_params &p, double bw, double tau)
+{
+    double g = (double)p.gamma_bins * bw;
+    if (p.gamma_schedule) g *= std::pow(10.0, (tau - 0.1) * 20.0 / 9.0 - 1.0);
+    return g;
+}
+
+ep_status ep::build_layout(const ep_netlist *nl, const ep_params *p, HostLayout &L)
+{
+    ep_status st = validate_netlist(nl, p);
+    if (st != EP_OK) return st;
+    L = HostLayout();
+    L.Mx = p->bins_x;
+    L.My = p->bins_y;
+    L.xl = nl->xl; L.yl = nl->yl; L.xh = nl->xh; L.yh = nl->yh;
+    L.bw = (float)(((double)nl->xh - nl->xl) / L.Mx);
+    L.bh = (float)(((double)nl->yh - nl->yl) / L.My);
+    L.sqrt2_bw = (float)(M_SQRT2 * (double)L.bw);
+    L.sqrt2_bh = (float)(M_SQRT2 * (double)L.bh);
+    L.n_mov = nl->n_mov;
+    L.n_fix = nl->n_fix;
 
     // ---- fillers (ePlace whitespace fillers)
     int32_t n_fill = 0;
     float fw = 0, fh = 0;
-    if ((st = ep_plan_fillers(nl, p, &n_fill, &fw, &fh, nullptr, nullptr, 0)) != EP_OK) {
-        destroy_ctx(c);
-        return st;
-    }
-    double mov_area = 0;
-    for (int k = 0; k < nl->n_mov; ++k) mov_area += (double)nl->w[k] * nl->h[k];
-    c->mov_area = mov_area;
-    c->n_fill = n_fill;
-    c->n_place = c->n_mov + c->n_fill;
-    c->n_all = c->n_place + c->n_fix;
+    if ((st = ep_plan_fillers(nl, p, &n_fill, &fw, &fh, nullptr, nullptr, 0)) != EP_OK) return st;
+    for (int k = 0; k < nl->n_mov; ++k) L.mov_area += (double)nl->w[k] * nl->h[k];
+    L.n_fill = n_fill;
+    L.n_place = L.n_mov + L.n_fill;
+    L.n_all = L.n_place + L.n_fix;
 
     // ---- host cell arrays [mov | fill | fix]
-    std::vector<float2> wh(c->n_all), pos(c->n_all);
-    for (int k = 0; k < c->n_mov; ++k) {
-        wh[k] = make_float2(nl->w[k], nl->h[k]);
-        pos[k] = make_float2(nl->x[k], nl->y[k]);
+    L.wh.resize(L.n_all);
+    L.pos.resize(L.n_all);
+    for (int k = 0; k < L.n_mov; ++k) {
+        L.wh[k] = make_float2(nl->w[k], nl->h[k]);
+        L.pos[k] = make_float2(nl->x[k], nl->y[k]);
     }
     {
-        std::vector<float> fxs(std::max(1, c->n_fill)), fys(std::max(1, c->n_fill));
+        std::vector<float> fxs(std::max(1, L.n_fill)), fys(std::max(1, L.n_fill));
         int32_t nf = 0;
-        ep_plan_fillers(nl, p, &nf, &fw, &fh, fxs.data(), fys.data(), c->n_fill);
-        for (int k = 0; k < c->n_fill; ++k) {
-            wh[c->n_mov + k] = make_float2(fw, fh);
-            pos[c->n_mov + k] = make_float2(fxs[k], fys[k]);
+        ep_plan_fillers(nl, p, &nf, &fw, &fh, fxs.data(), fys.data(), L.n_fill);
+        for (int k = 0; k < L.n_fill; ++k) {
+            L.wh[L.n_mov + k] = make_float2(fw, fh);
+            L.pos[L.n_mov + k] = make_float2(fxs[k], fys[k]);
         }
     }
-    for (int k = 0; k < c->n_fix; ++k) {
-        wh[c->n_place + k] = make_float2(nl->w[nl->n_mov + k], nl->h[nl->n_mov + k]);
-        pos[c->n_place + k] = make_float2(nl->x[nl->n_mov + k], nl->y[nl->n_mov + k]);
+    for (int k = 0; k < L.n_fix; ++k) {
+        L.wh[L.n_place + k] = make_float2(nl->w[nl->n_mov + k], nl->h[nl->n_mov + k]);
+        L.pos[L.n_place + k] = make_float2(nl->x[nl->n_mov + k], nl->y[nl->n_mov + k]);
     }
     // movable initial positions clamped into the die
-    for (int k = 0; k < c->n_place; ++k) {
-        pos[k].x = std::min(std::max(pos[k].x, c->xl), c->xh - wh[k].x);
-        pos[k].y = std::min(std::max(pos[k].y, c->yl), c->yh - wh[k].y);
+    for (int k = 0; k < L.n_place; ++k) {
+        L.pos[k].x = std::min(std::max(L.pos[k].x, L.xl), L.xh - L.wh[k].x);
+        L.pos[k].y = std::min(std::max(L.pos[k].y, L.yl), L.yh - L.wh[k].y);
     }
 
     // ---- nets regrouped by lane class (degree < 2 nets go last, untouched)
     const int N = nl->n_nets;
-    c->n_nets = N;
-    c->n_pins = nl->n_pins;
+    L.n_nets = N;
+    L.n_pins = nl->n_pins;
     const int lane_classes[kMaxBuckets] = {2, 4, 8, 16, 32, 64};
     std::vector<std::vector<int32_t>> cls(kMaxBuckets + 1);
     for (int n = 0; n < N; ++n) {
@@ -415,15 +404,15 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
             std::stable_sort(cls[b].begin(), cls[b].end(),
                              [&](int32_t a2, int32_t b2) { return key[a2] < key[b2]; });
     }
-    std::vector<int32_t> net_start(N + 1), pin_cell(nl->n_pins);
-    std::vector<float2> pin_off(nl->n_pins);
-    c->net_perm.clear();
-    c->net_perm.reserve(N);
+    L.net_start.assign(N + 1, 0);
+    L.pin_cell.assign(nl->n_pins, 0);
+    L.pin_off.assign(nl->n_pins, make_float2(0.f, 0.f));
+    L.net_perm.reserve(N);
     int pin_w = 0, net_w = 0, blk = 0;
-    c->buckets.n = 0;
+    L.buckets.n = 0;
     for (int b = 0; b <= kMaxBuckets; ++b) {
         if (b < kMaxBuckets && !cls[b].empty()) {
-            Bucket &bk = c->buckets.b[c->buckets.n++];
+            Bucket &bk = L.buckets.b[L.buckets.n++];
             bk.lanes = lane_classes[b];
             bk.net0 = net_w;
             bk.net1 = net_w + (int32_t)cls[b].size();
@@ -431,34 +420,75 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
             blk += (int)(((int64_t)cls[b].size() * bk.lanes + kBlock - 1) / kBlock);
         }
         for (int32_t n : cls[b]) {
-            net_start[net_w] = pin_w;
-            c->net_perm.push_back(n);
+            L.net_start[net_w] = pin_w;
+            L.net_perm.push_back(n);
             for (int q = nl->net_start[n]; q < nl->net_start[n + 1]; ++q) {
                 int32_t cell = nl->pin_cell[q];
-                if (cell >= c->n_mov) cell += c->n_fill;  // fixed cells follow the fillers
-                pin_cell[pin_w] = cell;
-                pin_off[pin_w] = make_float2(nl->pin_dx[q], nl->pin_dy[q]);
+                if (cell >= L.n_mov) cell += L.n_fill;  // fixed cells follow the fillers
+                L.pin_cell[pin_w] = cell;
+                L.pin_off[pin_w] = make_float2(nl->pin_dx[q], nl->pin_dy[q]);
                 ++pin_w;
             }
             ++net_w;
         }
     }
-    net_start[N] = pin_w;
-    c->wa_blocks = blk;
-    c->wa_part_blocks = std::min(blk, kWaGridCap);
+    L.net_start[N] = pin_w;
+    L.wa_blocks = blk;
 
     // ---- cell -> pin CSR (movable cells)
-    std::vector<int32_t> cps(c->n_mov + 1, 0);
+    L.cps.assign(L.n_mov + 1, 0);
     for (int q = 0; q < nl->n_pins; ++q)
-        if (pin_cell[q] < c->n_mov) cps[pin_cell[q] + 1]++;
-    for (int k = 0; k < c->n_mov; ++k) cps[k + 1] += cps[k];
-    c->n_pins_mov = cps[c->n_mov];
-    std::vector<int32_t> cpins(std::max(1, c->n_pins_mov)), fillp(cps.begin(), cps.end() - 1);
+        if (L.pin_cell[q] < L.n_mov) L.cps[L.pin_cell[q] + 1]++;
+    for (int k = 0; k < L.n_mov; ++k) L.cps[k + 1] += L.cps[k];
+    L.n_pins_mov = L.cps[L.n_mov];
+    L.cpins.assign(std::max(1, L.n_pins_mov), 0);
+    std::vector<int32_t> fillp(L.cps.begin(), L.cps.end() - 1);
     for (int q = 0; q < nl->n_pins; ++q)
-        if (pin_cell[q] < c->n_mov) cpins[fillp[pin_cell[q]]++] = q;
+        if (L.pin_cell[q] < L.n_mov) L.cpins[fillp[L.pin_cell[q]]++] = q;
 
     // ---- hpwl reference of the mu schedule: default scales with the netlist
-    if (!(c->prm.hpwl_ref > 0)) c->prm.hpwl_ref = kHpwlRefPerNetBin * (double)std::max(1, N) * c->bw;
+    L.hpwl_ref = p->hpwl_ref > 0 ? p->hpwl_ref : kHpwlRefPerNetBin * (double)std::max(1, N) * L.bw;
+    fixed_map(L, L.fixmap);
+    return EP_OK;
+}
+
+extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx **out)
+{
+    if (!out) return fail(EP_EINVAL, "ep_create: null output");
+    *out = nullptr;
+    HostLayout L;
+    ep_status st = build_layout(nl, p, L);
+    if (st != EP_OK) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(EP_EDEVICE, "ep_create: no HIP device available");
+
+    ep_ctx *c = new (std::nothrow) ep_ctx();
+    if (!c) return fail(EP_ENOMEM, "ep_create: host allocation failed");
+    c->prm = *p;
+    c->prm.hpwl_ref = L.hpwl_ref;
+    (void)hipGetDevice(&c->device);
+    c->Mx = L.Mx;
+    c->My = L.My;
+    c->xl = L.xl; c->yl = L.yl; c->xh = L.xh; c->yh = L.yh;
+    c->bw = L.bw;
+    c->bh = L.bh;
+    c->sqrt2_bw = L.sqrt2_bw;
+    c->sqrt2_bh = L.sqrt2_bh;
+    c->n_mov = L.n_mov;
+    c->n_fix = L.n_fix;
+    c->n_fill = L.n_fill;
+    c->n_place = L.n_place;
+    c->n_all = L.n_all;
+    c->mov_area = L.mov_area;
+    const int N = L.n_nets;
+    c->n_nets = N;
+    c->n_pins = L.n_pins;
+    c->n_pins_mov = L.n_pins_mov;
+    c->net_perm = L.net_perm;
+    c->buckets = L.buckets;
+    c->wa_blocks = L.wa_blocks;
+    c->wa_part_blocks = std::min(L.wa_blocks, kWaGridCap);
 
     // ---- device buffers
     hipError_t he = hipStreamCreate(&c->stream);
@@ -473,8 +503,6 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     c->part_ovf_off = 2 * c->wa_part_blocks;
     c->part_cell_off = c->part_ovf_off + 4 * c->ovf_blocks;
     c->part_total = c->part_cell_off + 7 * c->cell_blocks;
-    std::vector<float> fixmap;
-    fixed_map(*c, wh, pos, fixmap);
 #define EP_TRY(x)                 \
     do {                          \
         ep_status s_ = (x);       \
@@ -485,11 +513,11 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     } while (0)
     EP_TRY(dalloc(&c->wh, c->n_all));
     EP_TRY(dalloc(&c->net_start, N + 1));
-    EP_TRY(dalloc(&c->pin_cell, nl->n_pins));
-    EP_TRY(dalloc(&c->pin_off, nl->n_pins));
-    EP_TRY(dalloc(&c->pin_grad, nl->n_pins));
+    EP_TRY(dalloc(&c->pin_cell, L.n_pins));
+    EP_TRY(dalloc(&c->pin_off, L.n_pins));
+    EP_TRY(dalloc(&c->pin_grad, L.n_pins));
     EP_TRY(dalloc(&c->cell_pin_start, c->n_mov + 1));
-    EP_TRY(dalloc(&c->cell_pins, cpins.size()));
+    EP_TRY(dalloc(&c->cell_pins, L.cpins.size()));
     for (int i = 0; i < 2; ++i) {
         EP_TRY(dalloc(&c->V[i], c->n_all));
         EP_TRY(dalloc(&c->U[i], c->n_place));
@@ -515,22 +543,22 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     }
     std::memset(c->h_scal, 0, sizeof(DevScalars));
     std::memset(c->h_res, 0, sizeof(DevResult));
-    EP_TRY(upload(c->wh, wh.data(), wh.size()));
-    EP_TRY(upload(c->net_start, net_start.data(), net_start.size()));
-    EP_TRY(upload(c->pin_cell, pin_cell.data(), pin_cell.size()));
-    EP_TRY(upload(c->pin_off, pin_off.data(), pin_off.size()));
-    EP_TRY(upload(c->cell_pin_start, cps.data(), cps.size()));
-    EP_TRY(upload(c->cell_pins, cpins.data(), cpins.size()));
-    EP_TRY(upload(c->map_fix, fixmap.data(), fixmap.size()));
+    EP_TRY(upload(c->wh, L.wh.data(), L.wh.size()));
+    EP_TRY(upload(c->net_start, L.net_start.data(), L.net_start.size()));
+    EP_TRY(upload(c->pin_cell, L.pin_cell.data(), L.pin_cell.size()));
+    EP_TRY(upload(c->pin_off, L.pin_off.data(), L.pin_off.size()));
+    EP_TRY(upload(c->cell_pin_start, L.cps.data(), L.cps.size()));
+    EP_TRY(upload(c->cell_pins, L.cpins.data(), L.cpins.size()));
+    EP_TRY(upload(c->map_fix, L.fixmap.data(), L.fixmap.size()));
     for (int i = 0; i < 2; ++i) {
-        EP_TRY(upload(c->V[i], pos.data(), pos.size()));
-        EP_TRY(upload(c->U[i], pos.data(), (size_t)c->n_place));
+        EP_TRY(upload(c->V[i], L.pos.data(), L.pos.size()));
+        EP_TRY(upload(c->U[i], L.pos.data(), (size_t)c->n_place));
         if (hipMemset(c->G[i], 0, sizeof(float2) * std::max(1, c->n_place)) != hipSuccess) {
             destroy_ctx(c);
             return fail(EP_EDEVICE, "ep_create: hipMemset failed");
         }
     }
-    if (hipMemset(c->pin_grad, 0, sizeof(float2) * std::max(1, nl->n_pins)) != hipSuccess ||
+    if (hipMemset(c->pin_grad, 0, sizeof(float2) * std::max(1, L.n_pins)) != hipSuccess ||
         hipMemset(c->ex, 0, sizeof(float) * nb) != hipSuccess ||
         hipMemset(c->ey, 0, sizeof(float) * nb) != hipSuccess ||
         hipMemset(c->part, 0, sizeof(double) * c->part_total) != hipSuccess ||

@@ -1,0 +1,126 @@
+// ep_batch.h -- device data of the batched engine (include/eplace_batch.h).
+//
+// All instances share concatenated device arrays.  Instance i owns
+//   cells [cell_off, cell_off + n_all)   of wh, V[2], U[2], G[2], order
+//   bins  [map_off, map_off + M*M)       of map_field/map_mov/map_fix/ex/ey
+//   bins  [2*map_off, 2*map_off + 2*M*M) of the transform scratch tA/tB
+//   nets and pins: contiguous ranges of net_start/pin_cell/pin_off/pin_grad,
+//   with net_start holding global pin indices and pin_cell global cell indices
+//   sort tiles [i*kSortTiles, (i+1)*kSortTiles) of sort_count/sort_off.
+// Blocks of every kernel map to instances through prefix tables (the blocks
+// of instance i are [pre[i], pre[i+1])); blocks of a finished instance return
+// at once, so a replay costs only the unfinished instances' work.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "ep_ctx.h"
+
+namespace ep {
+
+constexpr int kBatchMaxInstances = 1024;
+constexpr int kBinsPerBlock = kBlock * 4;  // fill / overflow kernels
+
+struct BInst {
+    int32_t cell_off, n_mov, n_place, n_all;
+    int32_t map_off, M;
+    float xl, yl, xh, yh, bw, bh, sqrt2_bw, sqrt2_bh;
+    float rho_t, cap;      // target density, target density * bin area
+    double mov_area;
+    double hpwl_ref;       // mu-schedule reference
+    double init_disp;      // probe displacement (init_disp_bins * bw)
+    double gamma_base;     // gamma_bins * bw
+};
+
+// Optimiser state of one instance (device-resident; the control kernel is
+// the only writer during a run).
+struct BState {
+    double alpha;      // Lipschitz step estimate (alpha_hat of the last accepted trial)
+    double alpha_try;  // step of the pending trial
+    double ak, ak1, coef;
+    double lambda, gamma;
+    double prev_hpwl, last_hpwl, last_tau, last_wa;
+    double eps;        // probe displacement factor
+    int64_t evals;
+    int32_t cur;       // parity of the current V/U/G buffers
+    int32_t iteration, trial, done, error;
+    float f_alpha, f_coef, f_ginv, f_lambda, f_eps;  // values the kernels read
+    int32_t pad;
+};
+
+// Reductions of the last evaluation of one instance (FP64, fixed order).
+struct BRes {
+    double hpwl, wa, ovf, tau, dv2, dg2, sum_gw, sum_f, gnorm2, gmax;
+    int64_t nonfinite;
+};
+
+// Optimiser parameters shared by all instances.
+struct BCtl {
+    double lambda_eta, mu_max, mu_min, stop_overflow;
+    int32_t gamma_schedule, max_backtrack, max_iters, pad;
+};
+
+struct BBucket {
+    int32_t inst, lanes, net0, net1, blk0;
+};
+
+// Device pointers handed to every batched kernel (by value).
+struct BDev {
+    const BInst *inst;
+    BState *st;
+    BRes *res;
+    BCtl ctl;
+    int32_t n_inst;
+    const float2 *wh;
+    float2 *V[2], *U[2], *G[2];
+    const int32_t *net_start, *pin_cell;
+    const float2 *pin_off;
+    float2 *pin_grad;
+    const int32_t *cps, *cpins;  // global cell -> pin CSR (empty for non-movable cells)
+    float *map_field, *map_mov, *map_fix, *ex, *ey;
+    float *map_mean;             // [n_inst]
+    uint32_t *map_max;           // [n_inst] float bits of max |map - mean|
+    int32_t *order, *sort_count, *sort_off;
+    const int32_t *pre_cell;     // 256 cells per block
+    const int32_t *pre_sort;     // kBlock * kSortCellsPerThread cells per block
+    const int32_t *pre_dens;     // kBlock * density cells-per-thread cells per block
+    const int32_t *pre_bin;      // kBinsPerBlock bins per block
+    const int32_t *pre_wa;       // WA blocks of each instance (instance-major buckets)
+    const BBucket *buckets;
+    int32_t n_buckets;
+    double *part_wa, *part_ovf, *part_cell;  // per-block partials, indexed by block
+};
+
+enum BEvalAt { kAtCur = 0, kAtTrial = 1 };
+enum BCtlMode { kCtlGamma = 0, kCtlLambda = 1, kCtlProbe = 2, kCtlAlpha = 3, kCtlIterate = 4 };
+enum BMoveMode { kMoveTrial = 0, kMoveProbe = 1, kMoveInitU = 2 };
+
+// ---- batched launchers (k_batch.hip)
+void bl_move(const BDev &d, int mode, int cell_blocks, hipStream_t s);
+void bl_maps(const BDev &d, int at, int bin_blocks, int sort_blocks, int dens_blocks,
+             int dens_cpt, hipStream_t s);
+void bl_wa(const BDev &d, int at, int wa_blocks, hipStream_t s);
+void bl_cell_grad(const BDev &d, int at, int cell_blocks, hipStream_t s);
+void bl_reduce(const BDev &d, hipStream_t s);
+void bl_control(const BDev &d, int mode, hipStream_t s);
+bool bl_density_cpt_supported(int cpt);
+
+// ---- batched field solve (k_field.hip)
+struct BatchFieldInst {
+    int M;
+    float bw, bh;
+    const float *map_field;
+    float *tA, *tB, *ex, *ey;
+    const uint32_t *map_max;  // float bits of max |map - mean| (split FP16 scaling)
+    const float *map_mean;
+    const int32_t *skip;      // instance finished
+};
+struct BatchField;
+ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &inst);
+void batch_field_free(BatchField *bf);
+void launch_field_batch(BatchField *bf, int precision, hipStream_t s);
+
+}  // namespace ep

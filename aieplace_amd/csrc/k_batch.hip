@@ -1,0 +1,678 @@
+// k_batch.hip -- kernels of the batched engine: the single-instance gradient
+// evaluation (k_place.hip) re-tiled so that one launch covers every unfinished
+// instance, plus the per-instance FP64 reduction and the device-side Nesterov
+// control (backtracking decision, lambda/gamma schedules) that replaces the
+// host optimiser loop of ep_step.
+//
+// Blocks map to instances through prefix tables; per-block partial sums are
+// indexed by the global block id and reduced per instance in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "ep_batch.h"
+#include "ep_ctx.h"
+#include "ep_device.h"
+
+namespace ep {
+
+namespace {
+
+using namespace dev;
+
+// Instance owning global block b: the largest i with pre[i] <= b.
+__device__ inline int find_inst(const int32_t *__restrict__ pre, int n, int b)
+{
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline GeomArgs inst_geom(const BInst &I)
+{
+    GeomArgs g;
+    g.v[0] = I.xl; g.v[1] = I.yl; g.v[2] = I.xh; g.v[3] = I.yh;
+    g.v[4] = I.bw; g.v[5] = I.bh; g.v[6] = I.sqrt2_bw; g.v[7] = I.sqrt2_bh;
+    return g;
+}
+
+template <class T>
+__device__ inline T pick(T p0, T p1, int k)
+{
+    return k ? p1 : p0;
+}
+
+// Position array an evaluation reads: V[cur] or the trial point V[cur ^ 1].
+__device__ inline const float2 *eval_pos(const BDev &d, int cur, int at)
+{
+    return pick<const float2 *>(d.V[0], d.V[1], cur ^ at);
+}
+
+// ------------------------------------------------------------------ moves
+
+// kMoveTrial: u' = clamp(v - alpha g), v' = clamp(u' + coef (u' - u)) into the
+// other parity; kMoveProbe: v_-1 = clamp(v - eps g); kMoveInitU: u = v.
+__global__ __launch_bounds__(kBlock) void kb_move(BDev d, int mode)
+{
+    const int i = find_inst(d.pre_cell, d.n_inst, blockIdx.x);
+    const BState *S = d.st + i;
+    if (S->done) return;
+    const BInst &I = d.inst[i];
+    const int lc = (blockIdx.x - d.pre_cell[i]) * kBlock + threadIdx.x;
+    if (lc >= I.n_place) return;
+    const int c = I.cell_off + lc;
+    const int a = S->cur;
+    const GeomArgs g = inst_geom(I);
+    const float2 s = d.wh[c];
+    const float2 vv = pick<const float2 *>(d.V[0], d.V[1], a)[c];
+    if (mode == kMoveTrial) {
+        const float alpha = S->f_alpha, coef = S->f_coef;
+        const float2 uu = pick<const float2 *>(d.U[0], d.U[1], a)[c];
+        const float2 gg = pick<const float2 *>(d.G[0], d.G[1], a)[c];
+        const float2 un = clamp_cell(make_float2(vv.x - alpha * gg.x, vv.y - alpha * gg.y), s, g);
+        const float2 vn =
+            clamp_cell(make_float2(un.x + coef * (un.x - uu.x), un.y + coef * (un.y - uu.y)), s, g);
+        pick(d.U[0], d.U[1], a ^ 1)[c] = un;
+        pick(d.V[0], d.V[1], a ^ 1)[c] = vn;
+    } else if (mode == kMoveProbe) {
+        const float eps = S->f_eps;
+        const float2 gg = pick<const float2 *>(d.G[0], d.G[1], a)[c];
+        pick(d.V[0], d.V[1], a ^ 1)[c] = clamp_cell(make_float2(vv.x - eps * gg.x, vv.y - eps * gg.y), s, g);
+    } else {
+        pick(d.U[0], d.U[1], a)[c] = vv;
+    }
+}
+
+// ------------------------------------------------------------------ maps
+
+__global__ __launch_bounds__(kBlock) void kb_fill(BDev d)
+{
+    const int i = find_inst(d.pre_bin, d.n_inst, blockIdx.x);
+    if (d.st[i].done) return;
+    const BInst &I = d.inst[i];
+    const int nb = I.M * I.M;
+    const int base = (blockIdx.x - d.pre_bin[i]) * kBinsPerBlock;
+#pragma unroll
+    for (int k = 0; k < kBinsPerBlock / kBlock; ++k) {
+        const int b = base + k * kBlock + threadIdx.x;
+        if (b < nb) {
+            const size_t o = (size_t)I.map_off + b;
+            d.map_field[o] = I.rho_t * d.map_fix[o];
+            d.map_mov[o] = 0.0f;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_sort_count(BDev d, int at)
+{
+    __shared__ int hist[kSortTiles];
+    const int i = find_inst(d.pre_sort, d.n_inst, blockIdx.x);
+    const BState *S = d.st + i;
+    if (S->done) return;
+    const BInst &I = d.inst[i];
+    const float2 *pos = eval_pos(d, S->cur, at);
+    const GeomArgs g = inst_geom(I);
+    for (int t = threadIdx.x; t < kSortTiles; t += kBlock) hist[t] = 0;
+    __syncthreads();
+    const int base = (blockIdx.x - d.pre_sort[i]) * kBlock * kSortCellsPerThread;
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k) {
+        const int idx = base + k * kBlock + threadIdx.x;
+        if (idx < I.n_place) {
+            const int c = I.cell_off + idx;
+            atomicAdd(&hist[cell_key(pos[c], d.wh[c], g, I.M, I.M)], 1);
+        }
+    }
+    __syncthreads();
+    int32_t *count = d.sort_count + (size_t)i * kSortTiles;
+    for (int t = threadIdx.x; t < kSortTiles; t += kBlock)
+        if (hist[t]) atomicAdd(&count[t], hist[t]);
+}
+
+// One block per instance: exclusive scan of the tile counts; clears the counts.
+__global__ __launch_bounds__(1024) void kb_sort_scan(BDev d)
+{
+    const int i = blockIdx.x;
+    if (d.st[i].done) return;
+    int32_t *count = d.sort_count + (size_t)i * kSortTiles;
+    int32_t *off = d.sort_off + (size_t)i * kSortTiles;
+    constexpr int per = kSortTiles / 1024;
+    __shared__ int part[1024];
+    int v[per], s = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        v[k] = count[threadIdx.x * per + k];
+        s += v[k];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int dd = 1; dd < 1024; dd <<= 1) {
+        const int add = threadIdx.x >= dd ? part[threadIdx.x - dd] : 0;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - s;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        off[threadIdx.x * per + k] = run;
+        run += v[k];
+        count[threadIdx.x * per + k] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_sort_scatter(BDev d, int at)
+{
+    __shared__ int hist[kSortTiles];
+    __shared__ int base_of[kSortTiles];
+    const int i = find_inst(d.pre_sort, d.n_inst, blockIdx.x);
+    const BState *S = d.st + i;
+    if (S->done) return;
+    const BInst &I = d.inst[i];
+    const float2 *pos = eval_pos(d, S->cur, at);
+    const GeomArgs g = inst_geom(I);
+    for (int t = threadIdx.x; t < kSortTiles; t += kBlock) hist[t] = 0;
+    __syncthreads();
+    const int base = (blockIdx.x - d.pre_sort[i]) * kBlock * kSortCellsPerThread;
+    int key[kSortCellsPerThread], rank[kSortCellsPerThread];
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k) {
+        const int idx = base + k * kBlock + threadIdx.x;
+        key[k] = -1;
+        if (idx < I.n_place) {
+            const int c = I.cell_off + idx;
+            key[k] = cell_key(pos[c], d.wh[c], g, I.M, I.M);
+            rank[k] = atomicAdd(&hist[key[k]], 1);
+        }
+    }
+    __syncthreads();
+    int32_t *off = d.sort_off + (size_t)i * kSortTiles;
+    for (int t = threadIdx.x; t < kSortTiles; t += kBlock)
+        if (hist[t]) base_of[t] = atomicAdd(&off[t], hist[t]);
+    __syncthreads();
+    int32_t *order = d.order + I.cell_off;
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k)
+        if (key[k] >= 0) order[base_of[key[k]] + rank[k]] = I.cell_off + base + k * kBlock + threadIdx.x;
+}
+
+// Density scatter of kBlock * CPT sorted cells per block (LDS tile when their
+// bin window fits, else global atomics), as k_density.
+template <int CPT>
+__global__ __launch_bounds__(kBlock) void kb_density(BDev d, int at)
+{
+    __shared__ float tile[2][kTile];
+    __shared__ int box[kBlock / 64][4];
+    const int i = find_inst(d.pre_dens, d.n_inst, blockIdx.x);
+    const BState *S = d.st + i;
+    if (S->done) return;
+    const BInst &I = d.inst[i];
+    const float2 *pos = eval_pos(d, S->cur, at);
+    const GeomArgs g = inst_geom(I);
+    const int M = I.M;
+    float *map_field = d.map_field + I.map_off, *map_mov = d.map_mov + I.map_off;
+    const int32_t *order = d.order + I.cell_off;
+    const int base = (blockIdx.x - d.pre_dens[i]) * kBlock * CPT;
+    const float area_scale = g.v[4] * g.v[5];
+    const int n_mov_end = I.cell_off + I.n_mov;
+    int bi0 = INT_MAX, bi1 = -1, bj0 = INT_MAX, bj1 = -1;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int idx = base + k * kBlock + threadIdx.x;
+        if (idx >= I.n_place) break;
+        const int c = order[idx];
+        const float2 p = pos[c], s = d.wh[c];
+        Rect r;
+        stretched_rect(p, s, g, M, M, r);
+        if (r.i0 <= r.i1 && r.j0 <= r.j1) {
+            bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
+            bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
+        }
+        if (c < n_mov_end) {
+            exact_rect(p, s, g, M, M, r);
+            if (r.i0 <= r.i1 && r.j0 <= r.j1) {
+                bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
+                bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
+            }
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        bi0 = min(bi0, __shfl_xor(bi0, m, 64));
+        bi1 = max(bi1, __shfl_xor(bi1, m, 64));
+        bj0 = min(bj0, __shfl_xor(bj0, m, 64));
+        bj1 = max(bj1, __shfl_xor(bj1, m, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        box[threadIdx.x >> 6][0] = bi0;
+        box[threadIdx.x >> 6][1] = bi1;
+        box[threadIdx.x >> 6][2] = bj0;
+        box[threadIdx.x >> 6][3] = bj1;
+    }
+    __syncthreads();
+    bi0 = box[0][0]; bi1 = box[0][1]; bj0 = box[0][2]; bj1 = box[0][3];
+    for (int w = 1; w < kBlock / 64; ++w) {
+        bi0 = min(bi0, box[w][0]); bi1 = max(bi1, box[w][1]);
+        bj0 = min(bj0, box[w][2]); bj1 = max(bj1, box[w][3]);
+    }
+    if (bi1 < bi0) return;  // whole block off-grid (uniform)
+    const int tw = bj1 - bj0 + 1;
+    const int tsz = (bi1 - bi0 + 1) * tw;
+    const bool lds = tsz <= kTile;
+    if (lds) {
+        for (int t = threadIdx.x; t < tsz; t += kBlock) {
+            tile[0][t] = 0.0f;
+            tile[1][t] = 0.0f;
+        }
+        __syncthreads();
+    }
+#pragma unroll 1
+    for (int k = 0; k < CPT; ++k) {
+        const int idx = base + k * kBlock + threadIdx.x;
+        if (idx >= I.n_place) break;
+        const int c = order[idx];
+        const float2 p = pos[c], s = d.wh[c];
+        Rect rs;
+        stretched_rect(p, s, g, M, M, rs);
+        const float sc = rs.ratio * area_scale;
+        for (int bi = rs.i0; bi <= rs.i1; ++bi) {
+            const float ox = ovl(rs.tx0, rs.tx1, bi);
+            if (ox <= 0.0f) continue;
+            for (int bj = rs.j0; bj <= rs.j1; ++bj) {
+                const float oy = ovl(rs.ty0, rs.ty1, bj);
+                if (oy <= 0.0f) continue;
+                const float v = ox * oy * sc;
+                if (lds)
+                    atomicAdd(&tile[0][(bi - bi0) * tw + (bj - bj0)], v);
+                else
+                    unsafeAtomicAdd(&map_field[(size_t)bi * M + bj], v);
+            }
+        }
+        if (c < n_mov_end) {
+            Rect re;
+            exact_rect(p, s, g, M, M, re);
+            for (int bi = re.i0; bi <= re.i1; ++bi) {
+                const float ox = ovl(re.tx0, re.tx1, bi);
+                if (ox <= 0.0f) continue;
+                for (int bj = re.j0; bj <= re.j1; ++bj) {
+                    const float oy = ovl(re.ty0, re.ty1, bj);
+                    if (oy <= 0.0f) continue;
+                    const float v = ox * oy * area_scale;
+                    if (lds)
+                        atomicAdd(&tile[1][(bi - bi0) * tw + (bj - bj0)], v);
+                    else
+                        unsafeAtomicAdd(&map_mov[(size_t)bi * M + bj], v);
+                }
+            }
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < tsz; t += kBlock) {
+            const int bi = bi0 + t / tw, bj = bj0 + t % tw;
+            const size_t b = (size_t)bi * M + bj;
+            const float a = tile[0][t], m = tile[1][t];
+            if (a != 0.0f) unsafeAtomicAdd(&map_field[b], a);
+            if (m != 0.0f) unsafeAtomicAdd(&map_mov[b], m);
+        }
+    }
+}
+
+// Overflow numerator and field-map sum/max/-min per block (as k_overflow).
+__global__ __launch_bounds__(kBlock) void kb_overflow(BDev d)
+{
+    const int i = find_inst(d.pre_bin, d.n_inst, blockIdx.x);
+    if (d.st[i].done) return;
+    const BInst &I = d.inst[i];
+    const int nb = I.M * I.M;
+    const int base = (blockIdx.x - d.pre_bin[i]) * kBinsPerBlock;
+    double acc[4] = {0.0, 0.0, -1e300, -1e300};
+#pragma unroll
+    for (int k = 0; k < kBinsPerBlock / kBlock; ++k) {
+        const int b = base + k * kBlock + threadIdx.x;
+        if (b < nb) {
+            const size_t o = (size_t)I.map_off + b;
+            const float v = d.map_mov[o] + I.rho_t * d.map_fix[o] - I.cap;
+            if (v > 0.0f) acc[0] += (double)v;
+            const double f = d.map_field[o];
+            acc[1] += f;
+            acc[2] = fmax(acc[2], f);
+            acc[3] = fmax(acc[3], -f);
+        }
+    }
+    block_reduce<4, 2>(acc, d.part_ovf + 4 * (size_t)blockIdx.x);
+}
+
+// One block per instance: map mean and max |map - mean| for the field solve.
+__global__ __launch_bounds__(kBlock) void kb_map_stats(BDev d)
+{
+    const int i = blockIdx.x;
+    if (d.st[i].done) return;
+    const BInst &I = d.inst[i];
+    double v[3] = {0.0, -1e300, -1e300};
+    for (int b = d.pre_bin[i] + threadIdx.x; b < d.pre_bin[i + 1]; b += kBlock) {
+        v[0] += d.part_ovf[4 * (size_t)b + 1];
+        v[1] = fmax(v[1], d.part_ovf[4 * (size_t)b + 2]);
+        v[2] = fmax(v[2], d.part_ovf[4 * (size_t)b + 3]);
+    }
+    __shared__ double out[3];
+    block_reduce<3, 2>(v, out);
+    if (threadIdx.x == 0) {
+        const double mean = out[0] / ((double)I.M * I.M);
+        d.map_mean[i] = (float)mean;
+        const double r = fmax(out[1] - mean, -out[2] + mean);
+        d.map_max[i] = __float_as_uint((float)fmax(r, 0.0));
+    }
+}
+
+// ------------------------------------------------------------------ WA gradient
+
+template <int G>
+__device__ inline void wa_tile(const BDev &d, const BBucket &bk, int blk, const float2 *pos,
+                               float ginv, double &hp, double &wa)
+{
+    const int net = bk.net0 + (blk * kBlock + (int)threadIdx.x) / G;
+    const int lane = threadIdx.x & (G - 1);
+    const bool active = net < bk.net1;
+    int s = 0, e = 0;
+    if (active) {
+        s = d.net_start[net];
+        e = d.net_start[net + 1];
+    }
+    wa_net<G>(s, e, active, lane, ginv, d.pin_cell, d.pin_off, pos, d.pin_grad, hp, wa);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_wa(BDev d, int at)
+{
+    int lo = 0, hi = d.n_buckets;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (d.buckets[mid].blk0 <= (int)blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const BBucket bk = d.buckets[lo];
+    const BState *S = d.st + bk.inst;
+    if (S->done) return;
+    const float2 *pos = eval_pos(d, S->cur, at);
+    const float ginv = S->f_ginv;
+    const int blk = blockIdx.x - bk.blk0;
+    double hp = 0.0, wa = 0.0;
+    switch (bk.lanes) {
+    case 2: wa_tile<2>(d, bk, blk, pos, ginv, hp, wa); break;
+    case 4: wa_tile<4>(d, bk, blk, pos, ginv, hp, wa); break;
+    case 8: wa_tile<8>(d, bk, blk, pos, ginv, hp, wa); break;
+    case 16: wa_tile<16>(d, bk, blk, pos, ginv, hp, wa); break;
+    case 32: wa_tile<32>(d, bk, blk, pos, ginv, hp, wa); break;
+    default: wa_tile<64>(d, bk, blk, pos, ginv, hp, wa); break;
+    }
+    double v[2] = {hp, wa};
+    block_reduce<2, 0>(v, d.part_wa + 2 * (size_t)blockIdx.x);
+}
+
+// ------------------------------------------------------------------ per-cell gather
+
+// One cell per thread (as k_cell_grad): g = (gW - lambda F) / max(1, pins + lambda area).
+// at == kAtTrial: position V[cur^1], references V[cur]/G[cur], output G[cur^1];
+// at == kAtCur:   position V[cur], no references, output G[cur].
+__global__ __launch_bounds__(kBlock) void kb_cell_grad(BDev d, int at)
+{
+    const int i = find_inst(d.pre_cell, d.n_inst, blockIdx.x);
+    const BState *S = d.st + i;
+    if (S->done) return;
+    const BInst &I = d.inst[i];
+    const int a = S->cur, b = a ^ at;
+    const float2 *pos = pick<const float2 *>(d.V[0], d.V[1], b);
+    float2 *gout = pick(d.G[0], d.G[1], b);
+    const float lambda = S->f_lambda;
+    const GeomArgs g = inst_geom(I);
+    const int M = I.M;
+    const float *ex = d.ex + I.map_off, *ey = d.ey + I.map_off;
+    const float area_scale = g.v[4] * g.v[5];
+    const int lc = (blockIdx.x - d.pre_cell[i]) * kBlock + threadIdx.x;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (lc < I.n_place) {
+        const int c = I.cell_off + lc;
+        const float2 p = pos[c], s = d.wh[c];
+        float gwx = 0.f, gwy = 0.f;
+        const int k0 = d.cps[c], k1 = d.cps[c + 1];
+        for (int k = k0; k < k1; ++k) {
+            const float2 pg = d.pin_grad[d.cpins[k]];
+            gwx += pg.x;
+            gwy += pg.y;
+        }
+        const int np = k1 - k0;
+        Rect r;
+        stretched_rect(p, s, g, M, M, r);
+        float fx = 0.f, fy = 0.f;
+        for (int bi = r.i0; bi <= r.i1; ++bi) {
+            const float ox = ovl(r.tx0, r.tx1, bi);
+            if (ox <= 0.0f) continue;
+            for (int bj = r.j0; bj <= r.j1; ++bj) {
+                const float oy = ovl(r.ty0, r.ty1, bj);
+                if (oy <= 0.0f) continue;
+                const size_t o = (size_t)bi * M + bj;
+                const float w = ox * oy;
+                fx += w * ex[o];
+                fy += w * ey[o];
+            }
+        }
+        const float fs = r.ratio * area_scale;
+        fx *= fs;
+        fy *= fs;
+        const float pre = fmaxf(1.0f, (float)np + lambda * s.x * s.y);
+        const float gx = (gwx - lambda * fx) / pre, gy = (gwy - lambda * fy) / pre;
+        gout[c] = make_float2(gx, gy);
+        if (at == kAtTrial) {
+            const float2 v0 = pick<const float2 *>(d.V[0], d.V[1], a)[c];
+            const double dx = (double)p.x - v0.x, dy = (double)p.y - v0.y;
+            acc[0] += dx * dx + dy * dy;
+            const float2 g0 = pick<const float2 *>(d.G[0], d.G[1], a)[c];
+            const double ddx = (double)gx - g0.x, ddy = (double)gy - g0.y;
+            acc[1] += ddx * ddx + ddy * ddy;
+        }
+        acc[2] += fabs((double)gwx) + fabs((double)gwy);
+        acc[3] += fabs((double)fx) + fabs((double)fy);
+        acc[4] += (double)gx * gx + (double)gy * gy;
+        if (!isfinite(gx) || !isfinite(gy)) acc[5] += 1.0;
+        acc[6] = fmax(fabs((double)gx), fabs((double)gy));
+    }
+    block_reduce<7, 1>(acc, d.part_cell + 7 * (size_t)blockIdx.x);
+}
+
+// ------------------------------------------------------------------ reduction + control
+
+// One block per instance: fixed-order FP64 sums of the instance's partials.
+__global__ __launch_bounds__(kBlock) void kb_reduce(BDev d)
+{
+    const int i = blockIdx.x;
+    if (d.st[i].done) return;
+    // hpwl, wa, ovf, dv2, dg2, |gw|, |f|, g2, nonfinite ; max |g|
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = d.pre_wa[i] + threadIdx.x; b < d.pre_wa[i + 1]; b += kBlock) {
+        v[0] += d.part_wa[2 * (size_t)b];
+        v[1] += d.part_wa[2 * (size_t)b + 1];
+    }
+    for (int b = d.pre_bin[i] + threadIdx.x; b < d.pre_bin[i + 1]; b += kBlock)
+        v[2] += d.part_ovf[4 * (size_t)b];
+    for (int b = d.pre_cell[i] + threadIdx.x; b < d.pre_cell[i + 1]; b += kBlock) {
+        const double *q = d.part_cell + 7 * (size_t)b;
+        v[3] += q[0]; v[4] += q[1]; v[5] += q[2]; v[6] += q[3]; v[7] += q[4]; v[8] += q[5];
+        v[9] = fmax(v[9], q[6]);
+    }
+    __shared__ double out[10];
+    block_reduce<10, 1>(v, out);
+    if (threadIdx.x == 0) {
+        const double mov_area = d.inst[i].mov_area;
+        BRes &R = d.res[i];
+        R.hpwl = out[0];
+        R.wa = out[1];
+        R.ovf = out[2];
+        R.tau = mov_area > 0 ? out[2] / mov_area : 0.0;
+        R.dv2 = out[3];
+        R.dg2 = out[4];
+        R.sum_gw = out[5];
+        R.sum_f = out[6];
+        R.gnorm2 = out[7];
+        R.nonfinite = (int64_t)out[8];
+        R.gmax = out[9];
+    }
+}
+
+__device__ inline double gamma_dev(const BInst &I, const BCtl &C, double tau)
+{
+    double g = I.gamma_base;
+    if (C.gamma_schedule) g *= pow(10.0, (tau - 0.1) * 20.0 / 9.0 - 1.0);
+    return g;
+}
+
+// Scalars of the coming iteration's first trial (ePlace Nesterov coefficients).
+__device__ inline void begin_iteration(BState &S)
+{
+    S.ak1 = 0.5 * (1.0 + sqrt(4.0 * S.ak * S.ak + 1.0));
+    S.coef = (S.ak - 1.0) / S.ak1;
+    S.alpha_try = S.alpha;
+    S.trial = 0;
+    S.f_alpha = (float)S.alpha;
+    S.f_coef = (float)S.coef;
+    S.f_ginv = (float)(1.0 / S.gamma);
+    S.f_lambda = (float)S.lambda;
+}
+
+// One thread per instance.  Initialisation steps and the per-trial decision of
+// ep_step: accept when alpha_hat >= 0.95 alpha_used (or after max_backtrack
+// trials), then lambda *= mu(dHPWL), gamma(tau), stop at overflow / max_iters.
+__global__ void kb_control(BDev d, int mode)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n_inst) return;
+    BState S = d.st[i];
+    if (S.done) return;
+    const BRes R = d.res[i];
+    const BInst &I = d.inst[i];
+    const BCtl &C = d.ctl;
+    switch (mode) {
+    case kCtlGamma:
+        S.gamma = gamma_dev(I, C, R.tau);
+        S.f_ginv = (float)(1.0 / S.gamma);
+        S.f_lambda = 0.0f;
+        break;
+    case kCtlLambda:
+        S.evals++;
+        S.lambda = R.sum_f > 0 ? C.lambda_eta * R.sum_gw / R.sum_f : C.lambda_eta;
+        S.prev_hpwl = S.last_hpwl = R.hpwl;
+        S.last_tau = R.tau;
+        S.last_wa = R.wa;
+        S.f_lambda = (float)S.lambda;
+        break;
+    case kCtlProbe:
+        if (R.nonfinite) {
+            S.error = 1;
+            S.done = 1;
+            break;
+        }
+        S.eps = R.gmax > 0 ? I.init_disp / R.gmax : 0.0;
+        S.f_eps = (float)S.eps;
+        break;
+    case kCtlAlpha: {
+        S.evals++;
+        const double num = sqrt(R.dv2), den = sqrt(R.dg2);
+        S.alpha = den > 0 ? num / den : S.eps;
+        if (!(S.alpha > 0) || !isfinite(S.alpha)) S.alpha = S.eps > 0 ? S.eps : 1e-3;
+        S.ak = 1.0;
+        S.iteration = 0;
+        begin_iteration(S);
+        break;
+    }
+    default: {
+        S.evals++;
+        if (R.nonfinite) {
+            S.error = 1;
+            S.done = 1;
+            break;
+        }
+        const double used = (double)S.f_alpha;
+        const double nn = sqrt(R.dv2), dd = sqrt(R.dg2);
+        const double ahat = dd > 0 ? nn / dd : used;
+        if (!(ahat >= 0.95 * used) && S.trial < C.max_backtrack - 1) {
+            S.alpha_try = ahat;
+            S.f_alpha = (float)ahat;
+            S.trial++;
+            break;
+        }
+        S.cur ^= 1;
+        S.alpha = ahat;
+        S.ak = S.ak1;
+        const double hp = R.hpwl, tau = R.tau;
+        S.last_hpwl = hp;
+        S.last_tau = tau;
+        S.last_wa = R.wa;
+        const double dh = hp - S.prev_hpwl;
+        double mu = dh < 0 ? C.mu_max : pow(C.mu_max, 1.0 - dh / I.hpwl_ref);
+        mu = fmin(fmax(mu, C.mu_min), C.mu_max);
+        S.lambda *= mu;
+        S.prev_hpwl = hp;
+        S.gamma = gamma_dev(I, C, tau);
+        S.iteration++;
+        if (tau <= C.stop_overflow || S.iteration >= C.max_iters) S.done = 1;
+        else begin_iteration(S);
+        break;
+    }
+    }
+    d.st[i] = S;
+}
+
+int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+
+bool bl_density_cpt_supported(int cpt) { return cpt == 4 || cpt == 8 || cpt == 16; }
+
+void bl_move(const BDev &d, int mode, int cell_blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_move, dim3(cell_blocks), dim3(kBlock), 0, s, d, mode);
+}
+
+void bl_maps(const BDev &d, int at, int bin_blocks, int sort_blocks, int dens_blocks, int dens_cpt,
+             hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_fill, dim3(bin_blocks), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(kb_sort_count, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
+    hipLaunchKernelGGL(kb_sort_scan, dim3(d.n_inst), dim3(1024), 0, s, d);
+    hipLaunchKernelGGL(kb_sort_scatter, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
+    if (dens_cpt == 4)
+        hipLaunchKernelGGL(kb_density<4>, dim3(dens_blocks), dim3(kBlock), 0, s, d, at);
+    else if (dens_cpt == 16)
+        hipLaunchKernelGGL(kb_density<16>, dim3(dens_blocks), dim3(kBlock), 0, s, d, at);
+    else
+        hipLaunchKernelGGL(kb_density<8>, dim3(dens_blocks), dim3(kBlock), 0, s, d, at);
+    hipLaunchKernelGGL(kb_overflow, dim3(bin_blocks), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(kb_map_stats, dim3(d.n_inst), dim3(kBlock), 0, s, d);
+}
+
+void bl_wa(const BDev &d, int at, int wa_blocks, hipStream_t s)
+{
+    if (wa_blocks > 0) hipLaunchKernelGGL(kb_wa, dim3(wa_blocks), dim3(kBlock), 0, s, d, at);
+}
+
+void bl_cell_grad(const BDev &d, int at, int cell_blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_cell_grad, dim3(cell_blocks), dim3(kBlock), 0, s, d, at);
+}
+
+void bl_reduce(const BDev &d, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_reduce, dim3(d.n_inst), dim3(kBlock), 0, s, d);
+}
+
+void bl_control(const BDev &d, int mode, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_control, dim3(ceil_div(d.n_inst, 64)), dim3(64), 0, s, d, mode);
+}
+
+}  // namespace ep

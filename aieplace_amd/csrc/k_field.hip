@@ -28,12 +28,14 @@
 // while it is staged into LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <new>
 #include <vector>
 
+#include "ep_batch.h"
 #include "ep_ctx.h"
 
 namespace ep {
@@ -61,6 +63,7 @@ struct GemmArgs {
     const uint32_t *bmax;       // FP16X3: max |data operand| (float bits)
     uint32_t *cmax;             // FP16X3: running max |output| of this product
     const float *bbias;         // nullable: constant subtracted from the data operand
+    const int32_t *skip;        // batched: nonzero when the instance has finished
 };
 
 __device__ inline float spectral_val(const GemmArgs &a, int row, int col, float v, float *o2)
@@ -109,13 +112,11 @@ constexpr int BK32 = 16;
 // 64x64 block tile, 4 waves of 32x32; global tiles [64][16] of A and Bt are
 // staged transposed into LDS ([k][m], [k][n]) for the 32x32x2 operand layout.
 template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs a)
+__device__ inline void gemm_f32_tile(const GemmArgs &a, int z, int m0, int n0)
 {
-    const int z = blockIdx.z;
     const float *__restrict__ A = a.A[z];
     const float *__restrict__ B = a.B[z];
     const int M = a.M, N = a.N, K = a.K;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     __shared__ float As[BK32][BM + 4];
     __shared__ float Bs[BK32][BN + 4];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -154,6 +155,12 @@ __global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs a)
         const int col = n0 + wn + (lane & 31);
         if (row < M && col < N) store_out(a, z, EPI, row, col, acc[i]);
     }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs a)
+{
+    gemm_f32_tile<EPI>(a, blockIdx.z, blockIdx.y * BM, blockIdx.x * BN);
 }
 
 // ---------------------------------------------------------------- split 16-bit
@@ -197,14 +204,12 @@ __device__ inline f32x16 mfma16(const uint16_t *pa, const uint16_t *pb, f32x16 a
 // three products each).  LDS rows are [m or n][k] with k contiguous, so each
 // lane reads its 8 consecutive k values with one 16-byte load.
 template <int EPI, bool F16>
-__global__ __launch_bounds__(256) void k_gemm_split(GemmArgs a)
+__device__ inline void gemm_split_tile(const GemmArgs &a, int z, int m0, int n0)
 {
-    const int z = blockIdx.z;
     const uint16_t *__restrict__ Ah = a.Ah[z];
     const uint16_t *__restrict__ Al = a.Al[z];
     const float *__restrict__ B = a.B[z];
     const int M = a.M, N = a.N, K = a.K;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     __shared__ __attribute__((aligned(16))) uint16_t sAh[BM][LDK];
     __shared__ __attribute__((aligned(16))) uint16_t sAl[BM][LDK];
     __shared__ __attribute__((aligned(16))) uint16_t sBh[BN][LDK];
@@ -276,6 +281,34 @@ __global__ __launch_bounds__(256) void k_gemm_split(GemmArgs a)
         if (row < M && col < N) m = fmaxf(m, store_out(a, z, EPI, row, col, acc[i] * inv));
     }
     if (F16 && a.cmax) track_max(a.cmax, m);
+}
+
+template <int EPI, bool F16>
+__global__ __launch_bounds__(256) void k_gemm_split(GemmArgs a)
+{
+    gemm_split_tile<EPI, F16>(a, blockIdx.z, blockIdx.y * BM, blockIdx.x * BN);
+}
+
+// Batched products: block b runs tile tiles[b] = (instance, m0, n0, z) with the
+// instance's argument block; finished instances are skipped (uniform per block).
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_f32_b(const GemmArgs *__restrict__ args,
+                                                    const int4 *__restrict__ tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    const GemmArgs &a = args[t.x];
+    if (a.skip && *a.skip) return;
+    gemm_f32_tile<EPI>(a, t.w, t.y, t.z);
+}
+
+template <int EPI, bool F16>
+__global__ __launch_bounds__(256) void k_gemm_split_b(const GemmArgs *__restrict__ args,
+                                                      const int4 *__restrict__ tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    const GemmArgs &a = args[t.x];
+    if (a.skip && *a.skip) return;
+    gemm_split_tile<EPI, F16>(a, t.w, t.y, t.z);
 }
 
 // ---------------------------------------------------------------- FP64 reference
@@ -626,6 +659,190 @@ ep_status field_error(Ctx &c, double *err_ex, double *err_ey)
     if (err_ex) *err_ex = m[1] > 0 ? m[0] / m[1] : 0.0;
     if (err_ey) *err_ey = m[3] > 0 ? m[2] / m[3] : 0.0;
     return EP_OK;
+}
+
+// ---------------------------------------------------------------- batched solve
+
+namespace {
+
+__global__ void k_clear_max_b(uint32_t *fmax, int n_inst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) {
+        fmax[4 * i + 1] = 0u;
+        fmax[4 * i + 2] = 0u;
+        fmax[4 * i + 3] = 0u;
+    }
+}
+
+struct MatSet {  // the six transform matrices of one grid size
+    float *f32[MAT_N] = {};
+    uint16_t *hi[MAT_N] = {}, *lo[MAT_N] = {}, *hi16[MAT_N] = {}, *lo16[MAT_N] = {};
+};
+
+}  // namespace
+
+struct BatchField {
+    std::vector<int> sizes;                  // distinct M
+    std::vector<MatSet> sets;
+    GemmArgs *d_args[3][4] = {};             // [precision][stage] per-instance argument blocks
+    int4 *d_tiles[4] = {};
+    int n_tiles[4] = {};
+    uint32_t *fmax = nullptr;                // [4 * n_inst]
+    int n_inst = 0;
+};
+
+ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &inst)
+{
+    BatchField *bf = new (std::nothrow) BatchField();
+    if (!bf) return fail(EP_ENOMEM, "batch field: host allocation failed");
+    out = bf;
+    bf->n_inst = (int)inst.size();
+    for (const BatchFieldInst &d : inst)
+        if (std::find(bf->sizes.begin(), bf->sizes.end(), d.M) == bf->sizes.end()) bf->sizes.push_back(d.M);
+    bf->sets.resize(bf->sizes.size());
+    for (size_t si = 0; si < bf->sizes.size(); ++si) {
+        const int M = bf->sizes[si];
+        for (int m = 0; m < MAT_N; ++m) {
+            const bool sine = (m == MAT_SYT || m == MAT_SXT);
+            const bool tr = !(m == MAT_CY || m == MAT_CX);
+            const std::vector<double> d = trig(M, sine, tr);
+            std::vector<float> f(d.size());
+            std::vector<uint16_t> h(d.size()), l(d.size()), h16(d.size()), l16(d.size());
+            for (size_t k = 0; k < d.size(); ++k) {
+                f[k] = (float)d[k];
+                h[k] = bf16_rne_host(f[k]);
+                l[k] = bf16_rne_host(f[k] - bf16_to_f_host(h[k]));
+                h16[k] = f16_host(f[k]);
+                l16[k] = f16_host(f[k] - f16_to_f_host(h16[k]));
+            }
+            MatSet &s = bf->sets[si];
+            if (hipMalloc(&s.f32[m], f.size() * 4) != hipSuccess || hipMalloc(&s.hi[m], h.size() * 2) != hipSuccess ||
+                hipMalloc(&s.lo[m], l.size() * 2) != hipSuccess || hipMalloc(&s.hi16[m], h.size() * 2) != hipSuccess ||
+                hipMalloc(&s.lo16[m], l.size() * 2) != hipSuccess)
+                return fail(EP_ENOMEM, "batch field: hipMalloc failed");
+            if (hipMemcpy(s.f32[m], f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(s.hi[m], h.data(), h.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(s.lo[m], l.data(), l.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(s.hi16[m], h16.data(), h16.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(s.lo16[m], l16.data(), l16.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(EP_EDEVICE, "batch field: upload failed");
+        }
+    }
+    const int n = bf->n_inst;
+    if (hipMalloc(&bf->fmax, sizeof(uint32_t) * 4 * std::max(1, n)) != hipSuccess ||
+        hipMemset(bf->fmax, 0, sizeof(uint32_t) * 4 * std::max(1, n)) != hipSuccess)
+        return fail(EP_ENOMEM, "batch field: hipMalloc failed");
+    // argument blocks per precision and stage, and the tile lists
+    const int precs[3] = {EP_PREC_FP32, EP_PREC_BF16X3, EP_PREC_FP16X3};
+    std::vector<int4> tiles[4];
+    for (int pi = 0; pi < 3; ++pi) {
+        const bool f16 = precs[pi] == EP_PREC_FP16X3;
+        std::vector<GemmArgs> st[4];
+        for (int i = 0; i < n; ++i) {
+            const BatchFieldInst &d = inst[i];
+            const int si = (int)(std::find(bf->sizes.begin(), bf->sizes.end(), d.M) - bf->sizes.begin());
+            const MatSet &s = bf->sets[si];
+            const size_t nb = (size_t)d.M * d.M;
+            uint32_t *fm = bf->fmax + 4 * i;
+            GemmArgs a{};
+            a.scale = (float)(4.0 / ((double)d.M * d.M) / ((double)d.bw * d.bh));
+            a.wxs = (float)(M_PI / d.M);
+            a.wys = (float)(M_PI / d.M * ((double)d.bw / d.bh));
+            a.M = a.N = a.K = d.M;
+            a.skip = d.skip;
+            auto setA = [&](GemmArgs &g, int z, int m) {
+                g.A[z] = s.f32[m];
+                g.Ah[z] = f16 ? s.hi16[m] : s.hi[m];
+                g.Al[z] = f16 ? s.lo16[m] : s.lo[m];
+            };
+            GemmArgs g1 = a;
+            setA(g1, 0, MAT_CY); g1.B[0] = d.map_field; g1.C[0] = d.tA;
+            g1.bmax = d.map_max; g1.cmax = f16 ? fm + 1 : nullptr; g1.bbias = d.map_mean;
+            GemmArgs g2 = a;
+            setA(g2, 0, MAT_CX); g2.B[0] = d.tA; g2.C[0] = d.tB; g2.C2[0] = d.tB + nb;
+            g2.bmax = fm + 1; g2.cmax = f16 ? fm + 2 : nullptr;
+            GemmArgs g3 = a;
+            setA(g3, 0, MAT_CYT); g3.B[0] = d.tB; g3.C[0] = d.tA;
+            setA(g3, 1, MAT_SYT); g3.B[1] = d.tB + nb; g3.C[1] = d.tA + nb;
+            g3.bmax = fm + 2; g3.cmax = f16 ? fm + 3 : nullptr;
+            GemmArgs g4 = a;
+            setA(g4, 0, MAT_SXT); g4.B[0] = d.tA; g4.C[0] = d.ex;
+            setA(g4, 1, MAT_CXT); g4.B[1] = d.tA + nb; g4.C[1] = d.ey;
+            g4.bmax = fm + 3; g4.cmax = nullptr;
+            st[0].push_back(g1);
+            st[1].push_back(g2);
+            st[2].push_back(g3);
+            st[3].push_back(g4);
+            if (pi == 0) {
+                const int nt = (d.M + BM - 1) / BM;
+                for (int stage = 0; stage < 4; ++stage)
+                    for (int z = 0; z < (stage >= 2 ? 2 : 1); ++z)
+                        for (int mt = 0; mt < nt; ++mt)
+                            for (int ntl = 0; ntl < nt; ++ntl) tiles[stage].push_back(make_int4(i, mt * BM, ntl * BN, z));
+            }
+        }
+        for (int stage = 0; stage < 4; ++stage) {
+            if (hipMalloc(&bf->d_args[pi][stage], sizeof(GemmArgs) * std::max<size_t>(1, st[stage].size())) != hipSuccess ||
+                hipMemcpy(bf->d_args[pi][stage], st[stage].data(), sizeof(GemmArgs) * st[stage].size(),
+                          hipMemcpyHostToDevice) != hipSuccess)
+                return fail(EP_ENOMEM, "batch field: argument upload failed");
+        }
+    }
+    for (int stage = 0; stage < 4; ++stage) {
+        bf->n_tiles[stage] = (int)tiles[stage].size();
+        if (hipMalloc(&bf->d_tiles[stage], sizeof(int4) * std::max<size_t>(1, tiles[stage].size())) != hipSuccess ||
+            hipMemcpy(bf->d_tiles[stage], tiles[stage].data(), sizeof(int4) * tiles[stage].size(),
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return fail(EP_ENOMEM, "batch field: tile upload failed");
+    }
+    return EP_OK;
+}
+
+void batch_field_free(BatchField *bf)
+{
+    if (!bf) return;
+    for (MatSet &s : bf->sets)
+        for (int m = 0; m < MAT_N; ++m) {
+            void *p[] = {s.f32[m], s.hi[m], s.lo[m], s.hi16[m], s.lo16[m]};
+            for (void *q : p)
+                if (q) (void)hipFree(q);
+        }
+    for (int pi = 0; pi < 3; ++pi)
+        for (int st = 0; st < 4; ++st)
+            if (bf->d_args[pi][st]) (void)hipFree(bf->d_args[pi][st]);
+    for (int st = 0; st < 4; ++st)
+        if (bf->d_tiles[st]) (void)hipFree(bf->d_tiles[st]);
+    if (bf->fmax) (void)hipFree(bf->fmax);
+    delete bf;
+}
+
+void launch_field_batch(BatchField *bf, int precision, hipStream_t s)
+{
+    const int pi = precision == EP_PREC_BF16X3 ? 1 : precision == EP_PREC_FP16X3 ? 2 : 0;
+    if (precision == EP_PREC_FP16X3)
+        hipLaunchKernelGGL(k_clear_max_b, dim3((bf->n_inst + 63) / 64), dim3(64), 0, s, bf->fmax, bf->n_inst);
+    for (int stage = 0; stage < 4; ++stage) {
+        const dim3 grid(bf->n_tiles[stage]);
+        const GemmArgs *args = bf->d_args[pi][stage];
+        const int4 *tiles = bf->d_tiles[stage];
+        if (precision == EP_PREC_FP32) {
+            if (stage == 1)
+                hipLaunchKernelGGL(k_gemm_f32_b<1>, grid, dim3(256), 0, s, args, tiles);
+            else
+                hipLaunchKernelGGL(k_gemm_f32_b<0>, grid, dim3(256), 0, s, args, tiles);
+        } else if (precision == EP_PREC_BF16X3) {
+            if (stage == 1)
+                hipLaunchKernelGGL((k_gemm_split_b<1, false>), grid, dim3(256), 0, s, args, tiles);
+            else
+                hipLaunchKernelGGL((k_gemm_split_b<0, false>), grid, dim3(256), 0, s, args, tiles);
+        } else {
+            if (stage == 1)
+                hipLaunchKernelGGL((k_gemm_split_b<1, true>), grid, dim3(256), 0, s, args, tiles);
+            else
+                hipLaunchKernelGGL((k_gemm_split_b<0, true>), grid, dim3(256), 0, s, args, tiles);
+        }
+    }
 }
 
 }  // namespace ep

@@ -1,0 +1,158 @@
+"""Batched engine (include/eplace_batch.h) against the single-instance path and the oracle.
+
+The batch runs the single-instance algorithm on every instance with
+device-side control, so per instance:
+  * the initial gamma, lambda_0 and preconditioned gradient equal the
+    single-instance values at the same point (FP32 rounding) and the FP64
+    oracle's gradient (2e-4 of its scale, the single-instance bar);
+  * a full run reaches overflow <= 0.10 with HPWL within 0.5% of the
+    single-instance run of the same instance (the trajectories differ only by
+    FP32 atomic summation order).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import aieplace_amd as A
+from aieplace_amd import api
+from aieplace_amd.batch import BatchPlacer
+from support import oracle_design
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_batch(counts, bins, **kw):
+    nls = [A.generate("stdcell", seed=k + 1, n_movable=c) for k, c in enumerate(counts)]
+    p = A.placer_params(A.CONFIGS["stdcell_200k"], bins_x=bins[0], bins_y=bins[0], **kw)
+    return nls, p
+
+
+def single_params(p, b):
+    q = api.default_params()
+    for name, _ in api.EpParams._fields_:
+        setattr(q, name, getattr(p, name))
+    q.bins_x = q.bins_y = b
+    return q
+
+
+def test_batch_init_matches_single_and_oracle(oracle):
+    counts, bins = [5_000, 9_000, 14_000], [128, 128, 256]
+    say(f"case batch init counts={counts} bins={bins}")
+    nls, p = small_batch(counts, bins)
+    with BatchPlacer(nls, bins, p) as bp:
+        bp.init()
+        for i, (nl, b) in enumerate(zip(nls, bins)):
+            st = bp.state(i)
+            gx, gy = bp.gradient(i)
+            q = single_params(p, b)
+            with A.Placer(nl, q) as pl:
+                tau0 = pl.overflow()
+                bw = (nl.die[2] - nl.die[0]) / b
+                gamma = 4.0 * bw * 10 ** ((tau0 - 0.1) * 20 / 9 - 1)
+                # FP32 bin width on the device side: agreement to ~1e-6
+                assert abs(st["gamma"] - gamma) <= 1e-5 * gamma, (i, st["gamma"], gamma)
+                g0 = pl.gradient(st["gamma"], 0.0)
+                sw = np.abs(g0["gwx"].astype(np.float64)).sum() + np.abs(g0["gwy"].astype(np.float64)).sum()
+                sf = np.abs(g0["fx"].astype(np.float64)).sum() + np.abs(g0["fy"].astype(np.float64)).sum()
+                lam0 = p.lambda_eta * sw / sf
+                assert abs(st["lambda"] - lam0) <= 1e-4 * lam0, (i, st["lambda"], lam0)
+                g = pl.gradient(st["gamma"], st["lambda"])
+                x, y = pl.get_positions()
+            for k, v in (("gx", gx), ("gy", gy)):
+                err = np.abs(v - g[k]).max() / np.abs(g[k]).max()
+                say(f"  inst {i} {k} batch vs single {err:.2e}")
+                assert err < 1e-4, (i, k, err)
+            d, _, _ = oracle_design(oracle, nl, q)
+            o = d.gradient(x.astype(np.float64), y.astype(np.float64), q.target_density,
+                           st["gamma"], st["lambda"])
+            for k, v in (("gx", gx), ("gy", gy)):
+                err = np.abs(v - o[k]).max() / np.abs(o[k]).max()
+                say(f"  inst {i} {k} batch vs oracle {err:.2e}")
+                assert err < 2e-4, (i, k, err)
+
+
+@pytest.mark.parametrize("precision", [api.EP_PREC_FP32, api.EP_PREC_BF16X3])
+def test_batch_trajectory_matches_single(precision):
+    """First 30 iterations: the device-side control takes the same decisions
+    (backtracking, lambda/gamma schedules) as ep_step, so HPWL, overflow and
+    the evaluation count agree with the single-instance run."""
+    counts, bins = [6_000, 11_000, 25_000, 8_000], [128, 128, 256, 128]
+    say(f"case batch trajectory precision={precision} counts={counts} bins={bins}")
+    nls, p = small_batch(counts, bins, precision=precision, max_iters=30)
+    with BatchPlacer(nls, bins, p) as bp:
+        info = bp.run()
+        res = bp.results()
+    assert info["finished"] == len(counts)
+    for i, (nl, b) in enumerate(zip(nls, bins)):
+        with A.Placer(nl, single_params(p, b)) as pl:
+            r = pl.run()
+        say(f"  inst {i}: batch it={res['iterations'][i]} ev={res['evals'][i]} hpwl={res['hpwl'][i]:.6e} "
+            f"ovf={res['overflow'][i]:.5f} | single it={r['iterations']} ev={r['evals']} "
+            f"hpwl={r['hpwl']:.6e} ovf={r['overflow']:.5f}")
+        assert res["iterations"][i] == r["iterations"] == 30
+        assert abs(int(res["evals"][i]) - r["evals"]) <= 2, i
+        assert abs(res["hpwl"][i] - r["hpwl"]) <= 1e-3 * r["hpwl"], i
+        assert abs(res["overflow"][i] - r["overflow"]) <= 1e-3, i
+
+
+def test_batch_run_matches_single():
+    """Full runs.  Every instance reaches overflow <= 0.10.  Final HPWL is
+    compared with the single-instance FP32 run; the bound is 0.5% or, for
+    instances whose trajectory amplifies rounding, 1.5x the single path's own
+    FP32-vs-split-BF16 spread on that instance (measured here)."""
+    counts, bins = [6_000, 11_000, 25_000, 8_000], [128, 128, 256, 128]
+    say(f"case batch run counts={counts} bins={bins}")
+    nls, p = small_batch(counts, bins)
+    with BatchPlacer(nls, bins, p) as bp:
+        info = bp.run()
+        res = bp.results()
+        say(f"  batch: {info}")
+        assert info["finished"] == len(counts)
+        # everything done: a further run launches nothing and moves nothing
+        x_before = [bp.positions(i)[0].copy() for i in range(len(counts))]
+        again = bp.run()
+        assert again["launches"] == 0
+        for i in range(len(counts)):
+            assert np.array_equal(bp.positions(i)[0], x_before[i])
+    for i, (nl, b) in enumerate(zip(nls, bins)):
+        with A.Placer(nl, single_params(p, b)) as pl:
+            r = pl.run()
+        q = single_params(p, b)
+        q.precision = api.EP_PREC_BF16X3
+        with A.Placer(nl, q) as pl:
+            r16 = pl.run()
+        spread = abs(r16["hpwl"] - r["hpwl"]) / r["hpwl"]
+        gap = abs(res["hpwl"][i] - r["hpwl"]) / r["hpwl"]
+        say(f"  inst {i}: batch it={res['iterations'][i]} hpwl={res['hpwl'][i]:.6e} "
+            f"ovf={res['overflow'][i]:.4f} | single it={r['iterations']} hpwl={r['hpwl']:.6e} "
+            f"| gap {gap * 100:.3f}% single fp32/bf16 spread {spread * 100:.3f}%")
+        assert res["overflow"][i] <= p.stop_overflow + 1e-9
+        assert gap <= max(5e-3, 1.5 * spread), i
+        assert res["evals"][i] >= res["iterations"][i] + 2
+
+
+def test_batch_reset_restarts_from_initial_placement():
+    counts, bins = [5_000, 7_000], [128, 128]
+    say(f"case batch reset counts={counts}")
+    nls, p = small_batch(counts, bins)
+    with BatchPlacer(nls, bins, p) as bp:
+        x0 = [bp.positions(i) for i in range(2)]
+        first = bp.run()
+        moved = [not np.array_equal(bp.positions(i)[0], x0[i][0]) for i in range(2)]
+        assert all(moved)
+        bp.reset()
+        for i in range(2):
+            x, y = bp.positions(i)
+            assert np.array_equal(x, x0[i][0]) and np.array_equal(y, x0[i][1])
+        second = bp.run()
+        assert second["finished"] == 2
+        r1 = first["instance_iterations"]
+        r2 = second["instance_iterations"]
+        assert abs(r1 - r2) <= max(6, 0.05 * r1), (r1, r2)
+

@@ -152,3 +152,48 @@ def test_null_arguments():
     assert api.lib.ep_destroy(None) == api.EP_EINVAL
     assert api.lib.ep_step(None, None) == api.EP_EINVAL
     assert api.lib.ep_create(None, None, None) == api.EP_EINVAL
+
+
+def _batch_create(nls, bins, p):
+    structs = [nl.c_struct() for nl in nls]
+    arr = (C.POINTER(api.EpNetlist) * len(nls))(*[C.pointer(s) for s in structs])
+    b = np.asarray(bins, np.int32)
+    h = C.c_void_p()
+    rc = api.lib.ep_batch_create(arr, api.iptr(b), len(nls), C.byref(p), C.byref(h))
+    return rc, h
+
+
+def test_batch_validation_errors_before_device():
+    nls = [A.generate("stdcell", seed=s, n_movable=2_000) for s in (1, 2)]
+    p = A.placer_params(A.CONFIGS["stdcell_200k"])
+    rc, _ = _batch_create(nls, [128, 100], p)
+    assert rc == api.EP_EINVAL and b"instance 1" in api.lib.ep_last_error()
+    assert api.lib.ep_batch_create(None, None, 0, C.byref(p), C.byref(C.c_void_p())) == api.EP_EINVAL
+    q = A.placer_params(A.CONFIGS["stdcell_200k"], learned_step=1)
+    assert _batch_create(nls, [128, 128], q)[0] == api.EP_ENOTSUP
+    bad = A.generate("stdcell", seed=3, n_movable=2_000)
+    bad.pin_cell[7] = 10 ** 7
+    rc, _ = _batch_create([nls[0], bad], [128, 128], p)
+    assert rc == api.EP_EINVAL and b"instance 1" in api.lib.ep_last_error()
+    assert api.lib.ep_batch_destroy(None) == api.EP_EINVAL
+    assert api.lib.ep_batch_run(None, 1, None) == api.EP_EINVAL
+
+
+def test_batch_create_without_device_reports_edevice():
+    from conftest import gpu_available
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    nls = [A.generate("stdcell", seed=s, n_movable=2_000) for s in (1, 2)]
+    rc, _ = _batch_create(nls, [128, 128], A.placer_params(A.CONFIGS["stdcell_200k"]))
+    assert rc == api.EP_EDEVICE
+
+
+def test_batch_workload_sizes_log_uniform():
+    from aieplace_amd.batch import batch_cell_counts
+    from aieplace_amd.configs import bins_for_cells
+
+    counts = batch_cell_counts(64)
+    assert len(counts) == 64 and min(counts) >= 5_000 and max(counts) <= 150_000
+    logs = np.log(counts)
+    assert logs.max() - logs.min() > 0.8 * (np.log(150_000) - np.log(5_000))
+    assert {bins_for_cells(c) for c in counts} == {128, 256, 512}
