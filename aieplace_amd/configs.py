@@ -23,10 +23,16 @@ CONFIGS = {
 
 
 def bins_for_cells(n: int) -> int:
-    """Bin grid matched to instance size (batched form): 128/256/512."""
-    if n < 20_000:
+    """Bin grid matched to instance size (batched form): 128/256/512.
+
+    The 200k-cell workload runs at 512x512 bins, 0.76 cells per bin; the
+    thresholds are the geometric midpoints of that density between grid sizes
+    (0.76 * 128 * 256 ~ 25k and 0.76 * 256 * 512 ~ 100k cells), so every
+    instance sees 0.3-1.5 cells per bin.
+    """
+    if n < 25_000:
         return 128
-    if n < 60_000:
+    if n < 100_000:
         return 256
     return 512
 

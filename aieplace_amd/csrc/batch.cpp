@@ -25,7 +25,7 @@ namespace {
 
 constexpr int kTrialsPerLaunch = 4;   // Nesterov trials captured per graph replay
 constexpr int kCellAlign = 64;        // instance cell ranges start on this boundary
-constexpr int kDensCpt[3] = {4, 8, 16};
+constexpr int kDensCpt[3] = {2, 4, 8};
 constexpr int kTuneReps = 5;
 
 #define EPB_HIP(call)                                                                        \
@@ -77,7 +77,8 @@ struct ep_batch {
     float *map_field = nullptr, *map_mov = nullptr, *map_fix = nullptr, *ex = nullptr, *ey = nullptr;
     float *tA = nullptr, *tB = nullptr, *map_mean = nullptr;
     uint32_t *map_max = nullptr;
-    int32_t *order = nullptr, *sort_count = nullptr, *sort_off = nullptr;
+    int32_t *order = nullptr, *sort_count = nullptr, *sort_off = nullptr, *dens_tab = nullptr;
+    int32_t dens_wcap = 0;
     int32_t *pre_cell = nullptr, *pre_sort = nullptr, *pre_bin = nullptr, *pre_wa = nullptr;
     int32_t *pre_dens[3] = {};
     BBucket *buckets = nullptr;
@@ -126,6 +127,8 @@ struct ep_batch {
         d.order = order;
         d.sort_count = sort_count;
         d.sort_off = sort_off;
+        d.dens_tab = dens_tab;
+        d.dens_wcap = dens_wcap;
         d.pre_cell = pre_cell;
         d.pre_sort = pre_sort;
         d.pre_dens = pre_dens[dens_variant];
@@ -157,7 +160,7 @@ static void destroy_batch(ep_batch *b)
     void *ptrs[] = {b->d_inst, b->d_st, b->d_res, b->wh, b->V[0], b->V[1], b->U[0], b->U[1],
                     b->G[0], b->G[1], b->net_start, b->pin_cell, b->cps, b->cpins, b->pin_off,
                     b->pin_grad, b->map_field, b->map_mov, b->map_fix, b->ex, b->ey, b->tA, b->tB,
-                    b->map_mean, b->map_max, b->order, b->sort_count, b->sort_off, b->pre_cell,
+                    b->map_mean, b->map_max, b->order, b->sort_count, b->sort_off, b->dens_tab, b->pre_cell,
                     b->pre_sort, b->pre_bin, b->pre_wa, b->pre_dens[0], b->pre_dens[1],
                     b->pre_dens[2], b->buckets, b->part_wa, b->part_ovf, b->part_cell};
     for (void *p : ptrs)
@@ -288,6 +291,10 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
         I.hpwl_ref = L.hpwl_ref;
         I.init_disp = (double)p->init_disp_bins * L.bw;
         I.gamma_base = (double)p->gamma_bins * L.bw;
+        const DensPlan dp = make_dens_plan(L.Mx, L.My, L.bw, L.bh, L.max_sw, L.max_sh, 4);
+        I.dens_level = dp.level;
+        I.dens_ext = dp.ext;
+        b->dens_wcap = std::max(b->dens_wcap, dp.wcap);
         b->inst.push_back(I);
         b->n_fix.push_back(L.n_fix);
         const int64_t span = (L.n_all + kCellAlign - 1) / kCellAlign * kCellAlign;
@@ -321,9 +328,12 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
         pre_cell.push_back(pre_cell.back() + (int32_t)((L.n_place + kBlock - 1) / kBlock));
         pre_sort.push_back(pre_sort.back() + (int32_t)((L.n_place + kBlock * kSortCellsPerThread - 1) /
                                                        (kBlock * kSortCellsPerThread)));
-        for (int v = 0; v < 3; ++v)
-            pre_dens[v].push_back(pre_dens[v].back() +
-                                  (int32_t)((L.n_place + kBlock * kDensCpt[v] - 1) / (kBlock * kDensCpt[v])));
+        for (int v = 0; v < 3; ++v) {
+            DensPlan dv{};
+            dv.level = I.dens_level;
+            dv.cpt = kDensCpt[v];
+            pre_dens[v].push_back(pre_dens[v].back() + dens_grid(dv, L.n_place));
+        }
         pre_bin.push_back(pre_bin.back() + (int32_t)(((int64_t)bins[i] * bins[i] + kBinsPerBlock - 1) / kBinsPerBlock));
         wa_base += L.wa_blocks;
         pre_wa.push_back((int32_t)wa_base);
@@ -390,6 +400,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     EPB_TRY(dalloc(&b->order, nc));
     EPB_TRY(dalloc(&b->sort_count, (size_t)n * kSortTiles));
     EPB_TRY(dalloc(&b->sort_off, (size_t)n * kSortTiles));
+    EPB_TRY(dalloc(&b->dens_tab, (size_t)n * (kSortTiles + 1)));
     EPB_TRY(dalloc(&b->pre_cell, n + 1));
     EPB_TRY(dalloc(&b->pre_sort, n + 1));
     EPB_TRY(dalloc(&b->pre_bin, n + 1));

@@ -226,7 +226,7 @@ static void destroy_ctx(ep_ctx *c)
     void *ptrs[] = {c->wh, c->net_start, c->pin_cell, c->pin_off, c->pin_grad, c->cell_pin_start,
                     c->cell_pins, c->V[0], c->V[1], c->U[0], c->U[1], c->G[0], c->G[1],
                     c->map_field, c->map_mov, c->map_fix, c->ex, c->ey, c->tA, c->tB, c->part,
-                    c->order, c->sort_count, c->sort_off,
+                    c->order, c->sort_count, c->sort_off, c->dens_tab,
                     c->d_scal, c->d_res, c->d_mlp_w};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -368,8 +368,10 @@ ep_status ep::build_layout(const ep_netlist *nl, const ep_params *p, HostLayout 
         L.wh[L.n_place + k] = make_float2(nl->w[nl->n_mov + k], nl->h[nl->n_mov + k]);
         L.pos[L.n_place + k] = make_float2(nl->x[nl->n_mov + k], nl->y[nl->n_mov + k]);
     }
-    // movable initial positions clamped into the die
+    // movable initial positions clamped into the die; largest smoothed cell
     for (int k = 0; k < L.n_place; ++k) {
+        L.max_sw = std::max(L.max_sw, (double)std::max(L.wh[k].x, L.sqrt2_bw));
+        L.max_sh = std::max(L.max_sh, (double)std::max(L.wh[k].y, L.sqrt2_bh));
         L.pos[k].x = std::min(std::max(L.pos[k].x, L.xl), L.xh - L.wh[k].x);
         L.pos[k].y = std::min(std::max(L.pos[k].y, L.yl), L.yh - L.wh[k].y);
     }
@@ -489,6 +491,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     c->buckets = L.buckets;
     c->wa_blocks = L.wa_blocks;
     c->wa_part_blocks = std::min(L.wa_blocks, kWaGridCap);
+    c->dens = make_dens_plan(L.Mx, L.My, L.bw, L.bh, L.max_sw, L.max_sh, 4);
 
     // ---- device buffers
     hipError_t he = hipStreamCreate(&c->stream);
@@ -529,6 +532,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     EP_TRY(dalloc(&c->order, std::max(1, c->n_place)));
     EP_TRY(dalloc(&c->sort_count, kSortTiles));
     EP_TRY(dalloc(&c->sort_off, kSortTiles));
+    EP_TRY(dalloc(&c->dens_tab, kSortTiles + 1));
     EP_TRY(dalloc(&c->ex, nb));
     EP_TRY(dalloc(&c->ey, nb));
     EP_TRY(dalloc(&c->tA, 2 * nb));

@@ -27,6 +27,7 @@ constexpr int kBinsPerBlock = kBlock * 4;  // fill / overflow kernels
 struct BInst {
     int32_t cell_off, n_mov, n_place, n_all;
     int32_t map_off, M;
+    int32_t dens_level, dens_ext;  // density super-tile level and window extent (DensPlan)
     float xl, yl, xh, yh, bw, bh, sqrt2_bw, sqrt2_bh;
     float rho_t, cap;      // target density, target density * bin area
     double mov_area;
@@ -84,9 +85,11 @@ struct BDev {
     float *map_mean;             // [n_inst]
     uint32_t *map_max;           // [n_inst] float bits of max |map - mean|
     int32_t *order, *sort_count, *sort_off;
+    int32_t *dens_tab;           // [n_inst * (kSortTiles + 1)] density block tables
+    int32_t dens_wcap;           // LDS window capacity per map (bins), max over instances
     const int32_t *pre_cell;     // 256 cells per block
     const int32_t *pre_sort;     // kBlock * kSortCellsPerThread cells per block
-    const int32_t *pre_dens;     // kBlock * density cells-per-thread cells per block
+    const int32_t *pre_dens;     // density blocks per instance (dens_grid bound)
     const int32_t *pre_bin;      // kBinsPerBlock bins per block
     const int32_t *pre_wa;       // WA blocks of each instance (instance-major buckets)
     const BBucket *buckets;

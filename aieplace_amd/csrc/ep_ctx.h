@@ -39,6 +39,44 @@ constexpr size_t kMaxStepRecords = (size_t)1 << 21;
 constexpr double kHpwlRefPerNetBin = 0.05;
 constexpr int kMaxBuckets = 6;  // lanes per net: 2,4,8,16,32,64
 
+// Density scatter plan: super-tile level (2^level x 2^level coarse tiles per
+// block window), window extent beyond the super tile in bins, LDS window
+// capacity per map in bins, cells per thread.
+struct DensPlan {
+    int32_t level = 0, ext = 0, wcap = 0, cpt = 4;
+};
+
+constexpr int kDensMaxWin = 6400;  // LDS bins per map of the largest window
+
+// Plan for an Mx x My grid whose smoothed cells are at most max_w x max_h wide.
+inline DensPlan make_dens_plan(int Mx, int My, double bw, double bh, double max_w, double max_h,
+                               int cpt)
+{
+    DensPlan p;
+    const int m = Mx < My ? Mx : My;
+    p.level = 6;
+    for (int k = 0; k <= 6; ++k)
+        if (((int64_t)m << k) / kSortSide >= 32) {
+            p.level = k;
+            break;
+        }
+    const int ex = (int)(max_w / bw) + 2, ey = (int)(max_h / bh) + 2;
+    p.ext = ex > ey ? ex : ey;
+    const int64_t sx = (((int64_t)Mx << p.level) + kSortSide - 1) / kSortSide + 2 + p.ext;
+    const int64_t sy = (((int64_t)My << p.level) + kSortSide - 1) / kSortSide + 2 + p.ext;
+    const int64_t w = sx * sy;
+    p.wcap = (int)(w < kDensMaxWin ? w : kDensMaxWin);
+    p.cpt = cpt;
+    return p;
+}
+
+// Density grid: at most one partial block per super tile beyond n / K blocks.
+inline int dens_grid(const DensPlan &p, int64_t n_place)
+{
+    const int64_t K = (int64_t)kBlock * p.cpt;
+    return (int)((n_place + K - 1) / K + (kSortTiles >> (2 * p.level)));
+}
+
 struct Bucket {
     int32_t lanes;  // lanes per net
     int32_t net0, net1;
@@ -128,6 +166,8 @@ struct Ctx {
     float *map_field = nullptr, *map_mov = nullptr, *map_fix = nullptr;
     int32_t *order = nullptr;                        // cells in spatial (coarse tile) order
     int32_t *sort_count = nullptr, *sort_off = nullptr;  // kSortTiles each
+    int32_t *dens_tab = nullptr;                     // kSortTiles + 1: density block table
+    DensPlan dens{};
     float *ex = nullptr, *ey = nullptr;
     float *tA = nullptr, *tB = nullptr;         // transform scratch, 2*Mx*My each
     void *split_mats = nullptr;                 // host handle: FP32 + split transform matrices

@@ -13,8 +13,7 @@
 namespace ep {
 namespace dev {
 
-constexpr int kTile = 4096;             // bins per LDS-privatised tile (per map)
-constexpr int kDensCellsPerThread = 8;  // cells per density-scatter thread
+constexpr int kSortScanThreads = 1024;
 
 struct Geom {
     float xl, yl, xh, yh, bw, bh, sqrt2_bw, sqrt2_bh;
@@ -251,6 +250,205 @@ __device__ inline void wa_net(int s, int e, bool active, int lane, float ginv,
         if (lane == 0) {
             hp_acc += (double)(xmx - xmn) + (double)(ymx - ymn);
             wa_acc += (double)((xmx - xmn) + mxp - mxm) + (double)((ymx - ymn) + myp - mym);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ density scatter
+//
+// Cells are counting-sorted by the coarse tile of their smoothed rectangle's
+// lower-left bin (64 x 64 coarse tiles in Morton order).  A super tile of level
+// k is an aligned block of 2^k x 2^k coarse tiles, i.e. 4^k consecutive keys,
+// so its cells are one contiguous run of the sorted order.  Every cell of a
+// super tile starts inside the super tile's bins, so its bins lie in the
+// window [x0 - 1, x1 + ext) x [y0 - 1, y1 + ext), where ext bounds a smoothed
+// cell's extent in bins; the -1 absorbs FP32 rounding of the exact rectangle
+// against the smoothed one.  Blocks take up to K cells of one super tile and
+// accumulate the window in LDS.
+
+// Exclusive scan of the per-key counts (one block of kSortScanThreads):
+// off[key] = first slot of key, counts cleared, and the density block table
+// dtab[s] = first block of super tile s (level `level`, K cells per block),
+// dtab[NS] = total blocks.
+__device__ inline void sort_scan_block(int32_t *__restrict__ count, int32_t *__restrict__ off,
+                                       int32_t *__restrict__ dtab, int level, int K)
+{
+    constexpr int per = kSortTiles / kSortScanThreads;
+    __shared__ int part[kSortScanThreads];
+    __shared__ int so[kSortTiles + 1];
+    const int t = threadIdx.x;
+    int v[per], s = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        v[k] = count[t * per + k];
+        s += v[k];
+    }
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < kSortScanThreads; d <<= 1) {  // Hillis-Steele inclusive scan
+        const int add = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        off[t * per + k] = run;
+        so[t * per + k] = run;
+        run += v[k];
+        count[t * per + k] = 0;
+    }
+    if (t == kSortScanThreads - 1) so[kSortTiles] = run;
+    __syncthreads();
+    // blocks per super tile, then the same scan over super tiles
+    const int ns = kSortTiles >> (2 * level);
+    int nb[per], sb = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        const int st = t * per + k;
+        nb[k] = 0;
+        if (st < ns) {
+            const int cnt = so[(st + 1) << (2 * level)] - so[st << (2 * level)];
+            nb[k] = (cnt + K - 1) / K;
+        }
+        sb += nb[k];
+    }
+    __syncthreads();
+    part[t] = sb;
+    __syncthreads();
+    for (int d = 1; d < kSortScanThreads; d <<= 1) {
+        const int add = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    run = part[t] - sb;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        const int st = t * per + k;
+        if (st < ns) dtab[st] = run;
+        run += nb[k];
+    }
+    if (t == kSortScanThreads - 1) dtab[ns] = run;
+}
+
+// Super tile s of density block b (dtab as written by sort_scan_block).
+__device__ inline int dens_super_tile(const int32_t *__restrict__ dtab, int ns, int b)
+{
+    int lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (dtab[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Cells [c0, c1) of the sorted order in block `chunk` of super tile s.  After
+// the sort's scatter pass off[key] holds the END of key's run.
+__device__ inline void dens_cell_range(const int32_t *__restrict__ off, int s, int level, int chunk,
+                                       int K, int &c0, int &c1)
+{
+    const int key0 = s << (2 * level), key1 = (s + 1) << (2 * level);
+    const int start = key0 > 0 ? off[key0 - 1] : 0;
+    const int stop = off[key1 - 1];
+    c0 = start + chunk * K;
+    c1 = min(c0 + K, stop);
+}
+
+// Bin window of super tile s (see above).
+__device__ inline void super_window(int s, int level, int Mx, int My, int ext, int &x0, int &y0,
+                                    int &nx, int &ny)
+{
+    int sx = 0, sy = 0;
+    for (int b = 0; b < 6 - level; ++b) {
+        sx |= ((s >> (2 * b)) & 1) << b;
+        sy |= ((s >> (2 * b + 1)) & 1) << b;
+    }
+    const int cx0 = sx << level, cy0 = sy << level, side = 1 << level;
+    const int bx0 = (cx0 * Mx + kSortSide - 1) / kSortSide;
+    const int by0 = (cy0 * My + kSortSide - 1) / kSortSide;
+    const int bx1 = ((cx0 + side) * Mx + kSortSide - 1) / kSortSide;
+    const int by1 = ((cy0 + side) * My + kSortSide - 1) / kSortSide;
+    x0 = max(0, bx0 - 1);
+    y0 = max(0, by0 - 1);
+    nx = min(Mx, bx1 + ext) - x0;
+    ny = min(My, by1 + ext) - y0;
+}
+
+// Adds v at bin (i, j): into the LDS window when inside, else straight into HBM.
+__device__ inline void dens_add(float *__restrict__ win, float *__restrict__ map, int i, int j,
+                                int x0, int y0, int nx, int ny, int My, bool lds, float v)
+{
+    const unsigned li = (unsigned)(i - x0), lj = (unsigned)(j - y0);
+    if (lds && li < (unsigned)nx && lj < (unsigned)ny)
+        atomicAdd(&win[li * ny + lj], v);
+    else
+        unsafeAtomicAdd(&map[(size_t)i * My + j], v);
+}
+
+// Density scatter of cells order[c0, c1) (global cell ids; ids < mov_end are
+// movable and also enter map_mov) through the LDS window (x0, y0, nx, ny).
+// win holds 2 * wcap floats of dynamic LDS (field map, movable map).
+template <int CPT>
+__device__ inline void density_window(const int32_t *__restrict__ order, int c0, int c1, int mov_end,
+                                      const float2 *__restrict__ pos, const float2 *__restrict__ wh,
+                                      const GeomArgs &g, int Mx, int My, int x0, int y0, int nx,
+                                      int ny, float *__restrict__ map_field,
+                                      float *__restrict__ map_mov, float *win, int wcap)
+{
+    const int nw = nx * ny;
+    const bool lds = nw <= wcap;
+    float *win0 = win, *win1 = win + wcap;
+    if (lds) {
+        for (int t = threadIdx.x; t < nw; t += kBlock) {
+            win0[t] = 0.0f;
+            win1[t] = 0.0f;
+        }
+        __syncthreads();
+    }
+    const float area_scale = g.v[4] * g.v[5];
+#pragma unroll 1
+    for (int k = 0; k < CPT; ++k) {
+        const int idx = c0 + k * kBlock + threadIdx.x;
+        if (idx >= c1) break;
+        const int c = order[idx];
+        const float2 p = pos[c], s = wh[c];
+        Rect rs;
+        stretched_rect(p, s, g, Mx, My, rs);
+        const float sc = rs.ratio * area_scale;
+        for (int i = rs.i0; i <= rs.i1; ++i) {
+            const float ox = ovl(rs.tx0, rs.tx1, i);
+            if (ox <= 0.0f) continue;
+            for (int j = rs.j0; j <= rs.j1; ++j) {
+                const float oy = ovl(rs.ty0, rs.ty1, j);
+                if (oy <= 0.0f) continue;
+                dens_add(win0, map_field, i, j, x0, y0, nx, ny, My, lds, ox * oy * sc);
+            }
+        }
+        if (c < mov_end) {
+            Rect re;
+            exact_rect(p, s, g, Mx, My, re);
+            for (int i = re.i0; i <= re.i1; ++i) {
+                const float ox = ovl(re.tx0, re.tx1, i);
+                if (ox <= 0.0f) continue;
+                for (int j = re.j0; j <= re.j1; ++j) {
+                    const float oy = ovl(re.ty0, re.ty1, j);
+                    if (oy <= 0.0f) continue;
+                    dens_add(win1, map_mov, i, j, x0, y0, nx, ny, My, lds, ox * oy * area_scale);
+                }
+            }
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < nw; t += kBlock) {
+            const int i = x0 + t / ny, j = y0 + t % ny;
+            const size_t b = (size_t)i * My + j;
+            const float a = win0[t], m = win1[t];
+            if (a != 0.0f) unsafeAtomicAdd(&map_field[b], a);
+            if (m != 0.0f) unsafeAtomicAdd(&map_mov[b], m);
         }
     }
 }

@@ -29,6 +29,7 @@ struct HostLayout {
     std::vector<int32_t> cps, cpins;         // movable cell -> pin CSR
     std::vector<float> fixmap;               // [Mx * My] fixed-cell overlap area
     double hpwl_ref = 0;                     // mu-schedule reference (resolved default)
+    double max_sw = 0, max_sh = 0;           // largest smoothed placeable cell (max(w, sqrt2 bw), ...)
 };
 
 // Validates the netlist and parameters (same checks and messages as ep_create).

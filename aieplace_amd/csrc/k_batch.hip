@@ -134,36 +134,14 @@ __global__ __launch_bounds__(kBlock) void kb_sort_count(BDev d, int at)
         if (hist[t]) atomicAdd(&count[t], hist[t]);
 }
 
-// One block per instance: exclusive scan of the tile counts; clears the counts.
-__global__ __launch_bounds__(1024) void kb_sort_scan(BDev d)
+// One block per instance: exclusive scan of the tile counts (clears the
+// counts) and the instance's density block table for K cells per block.
+__global__ __launch_bounds__(kSortScanThreads) void kb_sort_scan(BDev d, int K)
 {
     const int i = blockIdx.x;
     if (d.st[i].done) return;
-    int32_t *count = d.sort_count + (size_t)i * kSortTiles;
-    int32_t *off = d.sort_off + (size_t)i * kSortTiles;
-    constexpr int per = kSortTiles / 1024;
-    __shared__ int part[1024];
-    int v[per], s = 0;
-#pragma unroll
-    for (int k = 0; k < per; ++k) {
-        v[k] = count[threadIdx.x * per + k];
-        s += v[k];
-    }
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int dd = 1; dd < 1024; dd <<= 1) {
-        const int add = threadIdx.x >= dd ? part[threadIdx.x - dd] : 0;
-        __syncthreads();
-        part[threadIdx.x] += add;
-        __syncthreads();
-    }
-    int run = part[threadIdx.x] - s;
-#pragma unroll
-    for (int k = 0; k < per; ++k) {
-        off[threadIdx.x * per + k] = run;
-        run += v[k];
-        count[threadIdx.x * per + k] = 0;
-    }
+    sort_scan_block(d.sort_count + (size_t)i * kSortTiles, d.sort_off + (size_t)i * kSortTiles,
+                    d.dens_tab + (size_t)i * (kSortTiles + 1), d.inst[i].dens_level, K);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_sort_scatter(BDev d, int at)
@@ -201,125 +179,28 @@ __global__ __launch_bounds__(kBlock) void kb_sort_scatter(BDev d, int at)
         if (key[k] >= 0) order[base_of[key[k]] + rank[k]] = I.cell_off + base + k * kBlock + threadIdx.x;
 }
 
-// Density scatter of kBlock * CPT sorted cells per block (LDS tile when their
-// bin window fits, else global atomics), as k_density.
+// Density scatter (as k_density): the instance's blocks take up to kBlock * CPT
+// cells of one super tile each and accumulate them in an LDS window.
 template <int CPT>
 __global__ __launch_bounds__(kBlock) void kb_density(BDev d, int at)
 {
-    __shared__ float tile[2][kTile];
-    __shared__ int box[kBlock / 64][4];
+    extern __shared__ float win[];
     const int i = find_inst(d.pre_dens, d.n_inst, blockIdx.x);
     const BState *S = d.st + i;
     if (S->done) return;
     const BInst &I = d.inst[i];
-    const float2 *pos = eval_pos(d, S->cur, at);
-    const GeomArgs g = inst_geom(I);
-    const int M = I.M;
-    float *map_field = d.map_field + I.map_off, *map_mov = d.map_mov + I.map_off;
-    const int32_t *order = d.order + I.cell_off;
-    const int base = (blockIdx.x - d.pre_dens[i]) * kBlock * CPT;
-    const float area_scale = g.v[4] * g.v[5];
-    const int n_mov_end = I.cell_off + I.n_mov;
-    int bi0 = INT_MAX, bi1 = -1, bj0 = INT_MAX, bj1 = -1;
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const int idx = base + k * kBlock + threadIdx.x;
-        if (idx >= I.n_place) break;
-        const int c = order[idx];
-        const float2 p = pos[c], s = d.wh[c];
-        Rect r;
-        stretched_rect(p, s, g, M, M, r);
-        if (r.i0 <= r.i1 && r.j0 <= r.j1) {
-            bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
-            bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
-        }
-        if (c < n_mov_end) {
-            exact_rect(p, s, g, M, M, r);
-            if (r.i0 <= r.i1 && r.j0 <= r.j1) {
-                bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
-                bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
-            }
-        }
-    }
-    for (int m = 32; m >= 1; m >>= 1) {
-        bi0 = min(bi0, __shfl_xor(bi0, m, 64));
-        bi1 = max(bi1, __shfl_xor(bi1, m, 64));
-        bj0 = min(bj0, __shfl_xor(bj0, m, 64));
-        bj1 = max(bj1, __shfl_xor(bj1, m, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        box[threadIdx.x >> 6][0] = bi0;
-        box[threadIdx.x >> 6][1] = bi1;
-        box[threadIdx.x >> 6][2] = bj0;
-        box[threadIdx.x >> 6][3] = bj1;
-    }
-    __syncthreads();
-    bi0 = box[0][0]; bi1 = box[0][1]; bj0 = box[0][2]; bj1 = box[0][3];
-    for (int w = 1; w < kBlock / 64; ++w) {
-        bi0 = min(bi0, box[w][0]); bi1 = max(bi1, box[w][1]);
-        bj0 = min(bj0, box[w][2]); bj1 = max(bj1, box[w][3]);
-    }
-    if (bi1 < bi0) return;  // whole block off-grid (uniform)
-    const int tw = bj1 - bj0 + 1;
-    const int tsz = (bi1 - bi0 + 1) * tw;
-    const bool lds = tsz <= kTile;
-    if (lds) {
-        for (int t = threadIdx.x; t < tsz; t += kBlock) {
-            tile[0][t] = 0.0f;
-            tile[1][t] = 0.0f;
-        }
-        __syncthreads();
-    }
-#pragma unroll 1
-    for (int k = 0; k < CPT; ++k) {
-        const int idx = base + k * kBlock + threadIdx.x;
-        if (idx >= I.n_place) break;
-        const int c = order[idx];
-        const float2 p = pos[c], s = d.wh[c];
-        Rect rs;
-        stretched_rect(p, s, g, M, M, rs);
-        const float sc = rs.ratio * area_scale;
-        for (int bi = rs.i0; bi <= rs.i1; ++bi) {
-            const float ox = ovl(rs.tx0, rs.tx1, bi);
-            if (ox <= 0.0f) continue;
-            for (int bj = rs.j0; bj <= rs.j1; ++bj) {
-                const float oy = ovl(rs.ty0, rs.ty1, bj);
-                if (oy <= 0.0f) continue;
-                const float v = ox * oy * sc;
-                if (lds)
-                    atomicAdd(&tile[0][(bi - bi0) * tw + (bj - bj0)], v);
-                else
-                    unsafeAtomicAdd(&map_field[(size_t)bi * M + bj], v);
-            }
-        }
-        if (c < n_mov_end) {
-            Rect re;
-            exact_rect(p, s, g, M, M, re);
-            for (int bi = re.i0; bi <= re.i1; ++bi) {
-                const float ox = ovl(re.tx0, re.tx1, bi);
-                if (ox <= 0.0f) continue;
-                for (int bj = re.j0; bj <= re.j1; ++bj) {
-                    const float oy = ovl(re.ty0, re.ty1, bj);
-                    if (oy <= 0.0f) continue;
-                    const float v = ox * oy * area_scale;
-                    if (lds)
-                        atomicAdd(&tile[1][(bi - bi0) * tw + (bj - bj0)], v);
-                    else
-                        unsafeAtomicAdd(&map_mov[(size_t)bi * M + bj], v);
-                }
-            }
-        }
-    }
-    if (lds) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < tsz; t += kBlock) {
-            const int bi = bi0 + t / tw, bj = bj0 + t % tw;
-            const size_t b = (size_t)bi * M + bj;
-            const float a = tile[0][t], m = tile[1][t];
-            if (a != 0.0f) unsafeAtomicAdd(&map_field[b], a);
-            if (m != 0.0f) unsafeAtomicAdd(&map_mov[b], m);
-        }
-    }
+    const int level = I.dens_level;
+    const int32_t *dtab = d.dens_tab + (size_t)i * (kSortTiles + 1);
+    const int ns = kSortTiles >> (2 * level);
+    const int b = blockIdx.x - d.pre_dens[i];
+    if (b >= dtab[ns]) return;
+    const int s = dens_super_tile(dtab, ns, b);
+    int c0, c1, x0, y0, nx, ny;
+    dens_cell_range(d.sort_off + (size_t)i * kSortTiles, s, level, b - dtab[s], kBlock * CPT, c0, c1);
+    super_window(s, level, I.M, I.M, I.dens_ext, x0, y0, nx, ny);
+    density_window<CPT>(d.order + I.cell_off, c0, c1, I.cell_off + I.n_mov, eval_pos(d, S->cur, at),
+                        d.wh, inst_geom(I), I.M, I.M, x0, y0, nx, ny, d.map_field + I.map_off,
+                        d.map_mov + I.map_off, win, d.dens_wcap);
 }
 
 // Overflow numerator and field-map sum/max/-min per block (as k_overflow).
@@ -631,7 +512,7 @@ int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 // ------------------------------------------------------------------ launchers
 
-bool bl_density_cpt_supported(int cpt) { return cpt == 4 || cpt == 8 || cpt == 16; }
+bool bl_density_cpt_supported(int cpt) { return cpt == 2 || cpt == 4 || cpt == 8; }
 
 void bl_move(const BDev &d, int mode, int cell_blocks, hipStream_t s)
 {
@@ -643,14 +524,15 @@ void bl_maps(const BDev &d, int at, int bin_blocks, int sort_blocks, int dens_bl
 {
     hipLaunchKernelGGL(kb_fill, dim3(bin_blocks), dim3(kBlock), 0, s, d);
     hipLaunchKernelGGL(kb_sort_count, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
-    hipLaunchKernelGGL(kb_sort_scan, dim3(d.n_inst), dim3(1024), 0, s, d);
+    hipLaunchKernelGGL(kb_sort_scan, dim3(d.n_inst), dim3(kSortScanThreads), 0, s, d, kBlock * dens_cpt);
     hipLaunchKernelGGL(kb_sort_scatter, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
-    if (dens_cpt == 4)
-        hipLaunchKernelGGL(kb_density<4>, dim3(dens_blocks), dim3(kBlock), 0, s, d, at);
-    else if (dens_cpt == 16)
-        hipLaunchKernelGGL(kb_density<16>, dim3(dens_blocks), dim3(kBlock), 0, s, d, at);
+    const size_t lds = 2 * sizeof(float) * d.dens_wcap;
+    if (dens_cpt == 2)
+        hipLaunchKernelGGL(kb_density<2>, dim3(dens_blocks), dim3(kBlock), lds, s, d, at);
+    else if (dens_cpt == 8)
+        hipLaunchKernelGGL(kb_density<8>, dim3(dens_blocks), dim3(kBlock), lds, s, d, at);
     else
-        hipLaunchKernelGGL(kb_density<8>, dim3(dens_blocks), dim3(kBlock), 0, s, d, at);
+        hipLaunchKernelGGL(kb_density<4>, dim3(dens_blocks), dim3(kBlock), lds, s, d, at);
     hipLaunchKernelGGL(kb_overflow, dim3(bin_blocks), dim3(kBlock), 0, s, d);
     hipLaunchKernelGGL(kb_map_stats, dim3(d.n_inst), dim3(kBlock), 0, s, d);
 }

@@ -57,33 +57,14 @@ __global__ __launch_bounds__(kBlock) void k_sort_count(const float2 *__restrict_
         if (hist[t]) atomicAdd(&count[t], hist[t]);
 }
 
-// Exclusive scan of the tile counts into offsets; clears the counts.
-__global__ __launch_bounds__(1024) void k_sort_scan(int32_t *__restrict__ count,
-                                                    int32_t *__restrict__ off)
+// Exclusive scan of the tile counts into offsets (clears the counts) and the
+// density block table of the coming scatter.
+__global__ __launch_bounds__(kSortScanThreads) void k_sort_scan(int32_t *__restrict__ count,
+                                                                int32_t *__restrict__ off,
+                                                                int32_t *__restrict__ dtab,
+                                                                int level, int K)
 {
-    constexpr int per = kSortTiles / 1024;
-    __shared__ int part[1024];
-    int v[per], s = 0;
-#pragma unroll
-    for (int k = 0; k < per; ++k) {
-        v[k] = count[threadIdx.x * per + k];
-        s += v[k];
-    }
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {  // Hillis-Steele inclusive scan
-        const int add = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
-        __syncthreads();
-        part[threadIdx.x] += add;
-        __syncthreads();
-    }
-    int run = part[threadIdx.x] - s;
-#pragma unroll
-    for (int k = 0; k < per; ++k) {
-        off[threadIdx.x * per + k] = run;
-        run += v[k];
-        count[threadIdx.x * per + k] = 0;
-    }
+    sort_scan_block(count, off, dtab, level, K);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(const float2 *__restrict__ pos,
@@ -116,122 +97,28 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const float2 *__restric
         if (key[k] >= 0) order[base_of[key[k]] + rank[k]] = base + k * kBlock + threadIdx.x;
 }
 
-// ---- density scatter.  Each block takes kBlock * kDensCellsPerThread cells of
-// the sorted order; if the union of their bin ranges fits in kTile bins the
-// block accumulates in LDS and flushes each non-zero bin with one global
-// atomic (coalesced along bin rows), otherwise it adds straight into the
-// global maps.
+// ---- density scatter: block b takes up to kBlock * CPT cells of one super
+// tile (dtab from k_sort_scan) and accumulates them in an LDS window.
+template <int CPT>
 __global__ __launch_bounds__(kBlock) void k_density(const int32_t *__restrict__ order,
-                                                    const float2 *__restrict__ pos,
-                                                    const float2 *__restrict__ wh, int n_place,
-                                                    int n_mov, GeomArgs g, int Mx, int My,
+                                                    const int32_t *__restrict__ off,
+                                                    const int32_t *__restrict__ dtab, int level,
+                                                    int ext, int wcap, const float2 *__restrict__ pos,
+                                                    const float2 *__restrict__ wh, int n_mov,
+                                                    GeomArgs g, int Mx, int My,
                                                     float *__restrict__ map_field,
                                                     float *__restrict__ map_mov)
 {
-    __shared__ float tile[2][kTile];
-    __shared__ int box[kBlock / 64][4];
-    const int base = blockIdx.x * kBlock * kDensCellsPerThread;
-    const float area_scale = g.v[4] * g.v[5];
-    int bi0 = INT_MAX, bi1 = -1, bj0 = INT_MAX, bj1 = -1;
-#pragma unroll
-    for (int k = 0; k < kDensCellsPerThread; ++k) {
-        const int idx = base + k * kBlock + threadIdx.x;
-        if (idx >= n_place) break;
-        const int c = order[idx];
-        const float2 p = pos[c], s = wh[c];
-        Rect r;
-        stretched_rect(p, s, g, Mx, My, r);
-        if (r.i0 <= r.i1 && r.j0 <= r.j1) {
-            bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
-            bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
-        }
-        if (c < n_mov) {
-            exact_rect(p, s, g, Mx, My, r);
-            if (r.i0 <= r.i1 && r.j0 <= r.j1) {
-                bi0 = min(bi0, r.i0); bi1 = max(bi1, r.i1);
-                bj0 = min(bj0, r.j0); bj1 = max(bj1, r.j1);
-            }
-        }
-    }
-    for (int m = 32; m >= 1; m >>= 1) {
-        bi0 = min(bi0, __shfl_xor(bi0, m, 64));
-        bi1 = max(bi1, __shfl_xor(bi1, m, 64));
-        bj0 = min(bj0, __shfl_xor(bj0, m, 64));
-        bj1 = max(bj1, __shfl_xor(bj1, m, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        box[threadIdx.x >> 6][0] = bi0;
-        box[threadIdx.x >> 6][1] = bi1;
-        box[threadIdx.x >> 6][2] = bj0;
-        box[threadIdx.x >> 6][3] = bj1;
-    }
-    __syncthreads();
-    bi0 = box[0][0]; bi1 = box[0][1]; bj0 = box[0][2]; bj1 = box[0][3];
-    for (int w = 1; w < kBlock / 64; ++w) {
-        bi0 = min(bi0, box[w][0]); bi1 = max(bi1, box[w][1]);
-        bj0 = min(bj0, box[w][2]); bj1 = max(bj1, box[w][3]);
-    }
-    if (bi1 < bi0) return;  // whole block off-grid (uniform)
-    const int tw = bj1 - bj0 + 1;
-    const int tsz = (bi1 - bi0 + 1) * tw;
-    const bool lds = tsz <= kTile;
-    if (lds) {
-        for (int t = threadIdx.x; t < tsz; t += kBlock) {
-            tile[0][t] = 0.0f;
-            tile[1][t] = 0.0f;
-        }
-        __syncthreads();
-    }
-#pragma unroll 1
-    for (int k = 0; k < kDensCellsPerThread; ++k) {
-        const int idx = base + k * kBlock + threadIdx.x;
-        if (idx >= n_place) break;
-        const int c = order[idx];
-        const float2 p = pos[c], s = wh[c];
-        Rect rs;
-        stretched_rect(p, s, g, Mx, My, rs);
-        const float sc = rs.ratio * area_scale;
-        for (int i = rs.i0; i <= rs.i1; ++i) {
-            const float ox = ovl(rs.tx0, rs.tx1, i);
-            if (ox <= 0.0f) continue;
-            for (int j = rs.j0; j <= rs.j1; ++j) {
-                const float oy = ovl(rs.ty0, rs.ty1, j);
-                if (oy <= 0.0f) continue;
-                const float v = ox * oy * sc;
-                if (lds)
-                    atomicAdd(&tile[0][(i - bi0) * tw + (j - bj0)], v);
-                else
-                    unsafeAtomicAdd(&map_field[(size_t)i * My + j], v);
-            }
-        }
-        if (c < n_mov) {
-            Rect re;
-            exact_rect(p, s, g, Mx, My, re);
-            for (int i = re.i0; i <= re.i1; ++i) {
-                const float ox = ovl(re.tx0, re.tx1, i);
-                if (ox <= 0.0f) continue;
-                for (int j = re.j0; j <= re.j1; ++j) {
-                    const float oy = ovl(re.ty0, re.ty1, j);
-                    if (oy <= 0.0f) continue;
-                    const float v = ox * oy * area_scale;
-                    if (lds)
-                        atomicAdd(&tile[1][(i - bi0) * tw + (j - bj0)], v);
-                    else
-                        unsafeAtomicAdd(&map_mov[(size_t)i * My + j], v);
-                }
-            }
-        }
-    }
-    if (lds) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < tsz; t += kBlock) {
-            const int i = bi0 + t / tw, j = bj0 + t % tw;
-            const size_t b = (size_t)i * My + j;
-            float a = tile[0][t], m = tile[1][t];
-            if (a != 0.0f) unsafeAtomicAdd(&map_field[b], a);
-            if (m != 0.0f) unsafeAtomicAdd(&map_mov[b], m);
-        }
-    }
+    extern __shared__ float win[];
+    const int ns = kSortTiles >> (2 * level);
+    const int b = blockIdx.x;
+    if (b >= dtab[ns]) return;
+    const int s = dens_super_tile(dtab, ns, b);
+    int c0, c1, x0, y0, nx, ny;
+    dens_cell_range(off, s, level, b - dtab[s], kBlock * CPT, c0, c1);
+    super_window(s, level, Mx, My, ext, x0, y0, nx, ny);
+    density_window<CPT>(order, c0, c1, n_mov, pos, wh, g, Mx, My, x0, y0, nx, ny, map_field,
+                        map_mov, win, wcap);
 }
 
 // Overflow numerator per block: sum_b max(0, mov_b + rho_t*fix_b - rho_t*A_b).
@@ -556,16 +443,25 @@ void launch_sort(Ctx &c, const float2 *pos, hipStream_t s)
     const int blocks = grid_for(c.n_place, kBlock * kSortCellsPerThread, INT_MAX);
     hipLaunchKernelGGL(k_sort_count, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
                        geom_args(c), c.Mx, c.My, c.sort_count);
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, c.sort_count, c.sort_off);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortScanThreads), 0, s, c.sort_count, c.sort_off,
+                       c.dens_tab, c.dens.level, kBlock * c.dens.cpt);
     hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
                        geom_args(c), c.Mx, c.My, c.sort_off, c.order);
 }
 
 void launch_density(Ctx &c, const float2 *pos, hipStream_t s)
 {
-    const int blocks = grid_for(c.n_place, kBlock * kDensCellsPerThread, INT_MAX);
-    hipLaunchKernelGGL(k_density, dim3(blocks), dim3(kBlock), 0, s, c.order, pos, c.wh, c.n_place,
-                       c.n_mov, geom_args(c), c.Mx, c.My, c.map_field, c.map_mov);
+    const DensPlan &dp = c.dens;
+    const int blocks = dens_grid(dp, c.n_place);
+    const size_t lds = 2 * sizeof(float) * dp.wcap;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, s, c.order, c.sort_off, c.dens_tab,
+                           dp.level, dp.ext, dp.wcap, pos, c.wh, c.n_mov, geom_args(c), c.Mx, c.My,
+                           c.map_field, c.map_mov);
+    };
+    if (dp.cpt == 2) go(k_density<2>);
+    else if (dp.cpt == 8) go(k_density<8>);
+    else go(k_density<4>);
 }
 
 void launch_overflow(Ctx &c, hipStream_t s)
