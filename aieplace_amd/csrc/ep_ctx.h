@@ -27,8 +27,8 @@ namespace ep {
 
 constexpr int kBlock = 256;
 // Launch caps of the reducing kernels (bounds the partial sums to finalize).
-constexpr int kWaGridCap = 2048;
-constexpr int kCellGridCap = 1024;
+constexpr int kWaGridCap = 16384;
+constexpr int kCellGridCap = 16384;
 // Spatial counting sort of cells: kSortSide x kSortSide coarse tiles.
 constexpr int kSortSide = 64;
 constexpr int kSortTiles = kSortSide * kSortSide;

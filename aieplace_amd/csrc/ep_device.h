@@ -409,12 +409,25 @@ __device__ inline void density_window(const int32_t *__restrict__ order, int c0,
         __syncthreads();
     }
     const float area_scale = g.v[4] * g.v[5];
-#pragma unroll 1
+    // all loads of the thread's cells first, so they are in flight together
+    int cid[CPT];
+    float2 cp[CPT], cs[CPT];
+#pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int idx = c0 + k * kBlock + threadIdx.x;
-        if (idx >= c1) break;
-        const int c = order[idx];
-        const float2 p = pos[c], s = wh[c];
+        cid[k] = idx < c1 ? order[idx] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+        if (cid[k] >= 0) {
+            cp[k] = pos[cid[k]];
+            cs[k] = wh[cid[k]];
+        }
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        if (cid[k] < 0) break;
+        const int c = cid[k];
+        const float2 p = cp[k], s = cs[k];
         Rect rs;
         stretched_rect(p, s, g, Mx, My, rs);
         const float sc = rs.ratio * area_scale;

@@ -461,6 +461,8 @@ void launch_density(Ctx &c, const float2 *pos, hipStream_t s)
     };
     if (dp.cpt == 2) go(k_density<2>);
     else if (dp.cpt == 8) go(k_density<8>);
+    else if (dp.cpt == 16) go(k_density<16>);
+    else if (dp.cpt == 32) go(k_density<32>);
     else go(k_density<4>);
 }
 

@@ -7,6 +7,10 @@ separately over --steps runs after --warmup untimed runs:
   mixed_learned   split-BF16 transforms + MLP-proposed step verified by the
                   backtracking condition (when built)
 `value` is the headline form's seconds (mixed_learned if present, else plain).
+The batched form (64 instances of 5k-150k cells, 128/256/512 bins, one graph
+replay per 4 Nesterov trials of all unfinished instances) is reported beside it
+as instance-iterations/s in FP32 and split-BF16 transforms, with the same
+instances run one at a time through the single-instance path on a sample.
 Prints ONE JSON line.  Runs on one GPU; --gpus > 1 exits with a message.
 """
 from __future__ import annotations
@@ -158,6 +162,52 @@ def cpu_baseline(A, nl, cfg, gpu_evals, budget_s):
                         f"form's {gpu_evals} evaluations to overflow<=0.10"))
 
 
+def batched_report(A, n=64, sample=16):
+    """Instance-iterations/s of the batched engine (FP32 and split-BF16 transforms)
+    and of the single-instance path on the first `sample` instances."""
+    from aieplace_amd import api
+    from aieplace_amd.batch import BatchPlacer, batch_workload
+    t0 = time.perf_counter()
+    nls, bins = batch_workload(n)
+    log(f"[batched] generated {n} instances ({sum(nl.n_mov for nl in nls)} movable cells) "
+        f"in {time.perf_counter() - t0:.1f}s")
+    cfg = A.CONFIGS["stdcell_200k"]
+    rep = dict(instances=n, movable_cells=int(sum(nl.n_mov for nl in nls)),
+               bins={str(b): bins.count(b) for b in sorted(set(bins))})
+    hpwl32 = None
+    for tag, prec in (("fp32", api.EP_PREC_FP32), ("bf16x3", api.EP_PREC_BF16X3)):
+        p = A.placer_params(cfg, precision=prec)
+        with BatchPlacer(nls, bins, p) as bp:
+            bp.run()                      # warm-up (graph capture, caches)
+            bp.reset()
+            r = bp.run()
+            res = bp.results()
+        rate = r["instance_iterations"] / r["seconds"]
+        log(f"[batched] {tag}: {r['instance_iterations']} instance-iterations in {r['seconds']:.3f}s "
+            f"= {rate:.0f}/s, {r['launches']} replays, density tile {r['density_cells_per_thread']} cells/thread")
+        rep[tag] = dict(instance_iterations_per_s=round(rate, 1), seconds=round(r["seconds"], 4),
+                        instance_iterations=int(r["instance_iterations"]), evals=int(r["evals"]),
+                        finished=int(r["finished"]), replays=int(r["launches"]),
+                        density_cells_per_thread=int(r["density_cells_per_thread"]),
+                        max_overflow=round(float(res["overflow"].max()), 4))
+        if prec == api.EP_PREC_FP32:
+            hpwl32 = res["hpwl"].copy()
+    tot_it, tot_s, gaps = 0, 0.0, []
+    for i in range(min(sample, n)):
+        q = A.placer_params(cfg, bins_x=bins[i], bins_y=bins[i], use_graph=1)
+        with A.Placer(nls[i], q) as pl:
+            r = pl.run()
+        tot_it += r["iterations"]
+        tot_s += r["seconds"]
+        gaps.append(abs(hpwl32[i] - r["hpwl"]) / r["hpwl"])
+    rep["single_instance_sample"] = dict(
+        instances=min(sample, n), instance_iterations_per_s=round(tot_it / tot_s, 1),
+        median_abs_hpwl_gap_vs_batch_pct=round(100 * float(np.median(gaps)), 3))
+    rep["speedup_vs_single_fp32"] = round(rep["fp32"]["instance_iterations_per_s"] /
+                                          rep["single_instance_sample"]["instance_iterations_per_s"], 2)
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +218,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision-report", action="store_true")
     ap.add_argument("--forms", default="plain,mixed,mixed_learned")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -230,6 +281,8 @@ def main():
     out["roofline"] = dict(kernel=dom["name"], **roof)
     out["kernels"] = {s["name"]: dict(avg_us=round(s["avg_us"], 2), launches=s["launches"])
                       for s in stats}
+    if not args.no_batched:
+        out["batched"] = batched_report(A)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, nl, cfg, plain["evals"], args.cpu_seconds)
         out["gpu_vs_cpu_plain"] = round(out["cpu_baseline"]["value"] / plain["seconds"], 1)

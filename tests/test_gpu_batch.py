@@ -82,7 +82,7 @@ def test_batch_trajectory_matches_single(precision):
     """First 30 iterations: the device-side control takes the same decisions
     (backtracking, lambda/gamma schedules) as ep_step, so HPWL, overflow and
     the evaluation count agree with the single-instance run."""
-    counts, bins = [6_000, 11_000, 25_000, 8_000], [128, 128, 256, 128]
+    counts, bins = [6_000, 11_000, 60_000, 8_000], [128, 128, 256, 128]
     say(f"case batch trajectory precision={precision} counts={counts} bins={bins}")
     nls, p = small_batch(counts, bins, precision=precision, max_iters=30)
     with BatchPlacer(nls, bins, p) as bp:
@@ -102,11 +102,11 @@ def test_batch_trajectory_matches_single(precision):
 
 
 def test_batch_run_matches_single():
-    """Full runs.  Every instance reaches overflow <= 0.10.  Final HPWL is
-    compared with the single-instance FP32 run; the bound is 0.5% or, for
-    instances whose trajectory amplifies rounding, 1.5x the single path's own
-    FP32-vs-split-BF16 spread on that instance (measured here)."""
-    counts, bins = [6_000, 11_000, 25_000, 8_000], [128, 128, 256, 128]
+    """Full runs.  Every instance reaches overflow <= 0.10, and its final HPWL
+    lies within 0.5% of the range the single-instance path gives for the same
+    instance over filler seeds 7-10 (the batch uses seed 7; final HPWL varies
+    with the filler placement and FP32 summation order)."""
+    counts, bins = [6_000, 11_000, 60_000, 8_000], [128, 128, 256, 128]
     say(f"case batch run counts={counts} bins={bins}")
     nls, p = small_batch(counts, bins)
     with BatchPlacer(nls, bins, p) as bp:
@@ -121,19 +121,20 @@ def test_batch_run_matches_single():
         for i in range(len(counts)):
             assert np.array_equal(bp.positions(i)[0], x_before[i])
     for i, (nl, b) in enumerate(zip(nls, bins)):
-        with A.Placer(nl, single_params(p, b)) as pl:
-            r = pl.run()
         q = single_params(p, b)
-        q.precision = api.EP_PREC_BF16X3
-        with A.Placer(nl, q) as pl:
-            r16 = pl.run()
-        spread = abs(r16["hpwl"] - r["hpwl"]) / r["hpwl"]
-        gap = abs(res["hpwl"][i] - r["hpwl"]) / r["hpwl"]
+        hp, its = [], []
+        for fs in (7, 8, 9, 10):
+            q.filler_seed = fs
+            with A.Placer(nl, q) as pl:
+                r = pl.run()
+            hp.append(r["hpwl"])
+            its.append(r["iterations"])
+        lo, hi = min(hp), max(hp)
         say(f"  inst {i}: batch it={res['iterations'][i]} hpwl={res['hpwl'][i]:.6e} "
-            f"ovf={res['overflow'][i]:.4f} | single it={r['iterations']} hpwl={r['hpwl']:.6e} "
-            f"| gap {gap * 100:.3f}% single fp32/bf16 spread {spread * 100:.3f}%")
+            f"ovf={res['overflow'][i]:.4f} | single over filler seeds: it {min(its)}..{max(its)} "
+            f"hpwl {lo:.6e}..{hi:.6e}")
         assert res["overflow"][i] <= p.stop_overflow + 1e-9
-        assert gap <= max(5e-3, 1.5 * spread), i
+        assert lo * 0.995 <= res["hpwl"][i] <= hi * 1.005, i
         assert res["evals"][i] >= res["iterations"][i] + 2
 
 
