@@ -104,17 +104,22 @@ def profile(pl, reps=20):
     return out
 
 
-def roofline_of(stats, precision_bf16=False):
+def roofline_of(stats, traffic=None, precision_bf16=False):
+    """Dominant item of ep_profile (longest call) against its roofline.
+    `traffic`: {item: measured HBM bytes per call} from the committed
+    rocprofv3 --pmc summary of the same workload, or None."""
     dom = max(stats, key=lambda s: s["avg_us"])
     sec = dom["avg_us"] * 1e-6
+    tr = (traffic or {}).get(dom["name"])
+    tr = None if tr is None else round(tr)
     if dom["flops"] > 0:
         peak = PEAK_BF16_MFMA_TFLOPS if precision_bf16 else PEAK_FP32_MFMA_TFLOPS
         ach = dom["flops"] / sec / 1e12
         return dom, dict(bound="mfma", achieved=round(ach, 3), peak=peak, unit="TFLOP/s",
-                         frac=round(ach / peak, 4), traffic=None)
+                         frac=round(ach / peak, 4), traffic=tr)
     ach = dom["bytes"] / sec / 1e9
     return dom, dict(bound="hbm", achieved=round(ach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                     frac=round(ach / PEAK_HBM_GBS, 4), traffic=None)
+                     frac=round(ach / PEAK_HBM_GBS, 4), traffic=tr)
 
 
 def cpu_baseline(A, nl, cfg, gpu_evals, budget_s):
@@ -276,9 +281,12 @@ def main():
 
     if not args.no_precision_report:
         out["field_error_vs_fp64"] = precision_report(A, nl, cfg)
+    from aieplace_amd.profiling import read_traffic
     stats = profile(placers["plain"])
-    dom, roof = roofline_of(stats)
+    dom, roof = roofline_of(stats, read_traffic(args.workload, api.EP_PREC_FP32))
     out["roofline"] = dict(kernel=dom["name"], **roof)
+    if roof["traffic"] is not None:
+        out["roofline"]["traffic_source"] = f"profiles/traffic_{args.workload}_fp32.md"
     out["kernels"] = {s["name"]: dict(avg_us=round(s["avg_us"], 2), launches=s["launches"])
                       for s in stats}
     if not args.no_batched:
