@@ -76,8 +76,10 @@ struct ep_batch {
     float2 *pin_off = nullptr, *pin_grad = nullptr;
     float *map_field = nullptr, *map_mov = nullptr, *map_fix = nullptr, *ex = nullptr, *ey = nullptr;
     float *tA = nullptr, *tB = nullptr, *map_mean = nullptr;
+    unsigned long long *acc_field = nullptr, *acc_mov = nullptr;
     uint32_t *map_max = nullptr;
-    int32_t *order = nullptr, *sort_count = nullptr, *sort_off = nullptr, *dens_tab = nullptr;
+    float4 *sorted = nullptr;
+    int32_t *sort_count = nullptr, *sort_off = nullptr, *dens_tab = nullptr;
     int32_t dens_wcap = 0;
     int32_t *pre_cell = nullptr, *pre_sort = nullptr, *pre_bin = nullptr, *pre_wa = nullptr;
     int32_t *pre_dens[3] = {};
@@ -120,11 +122,13 @@ struct ep_batch {
         d.map_field = map_field;
         d.map_mov = map_mov;
         d.map_fix = map_fix;
+        d.acc_field = acc_field;
+        d.acc_mov = acc_mov;
         d.ex = ex;
         d.ey = ey;
         d.map_mean = map_mean;
         d.map_max = map_max;
-        d.order = order;
+        d.sorted = sorted;
         d.sort_count = sort_count;
         d.sort_off = sort_off;
         d.dens_tab = dens_tab;
@@ -159,8 +163,8 @@ static void destroy_batch(ep_batch *b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     void *ptrs[] = {b->d_inst, b->d_st, b->d_res, b->wh, b->V[0], b->V[1], b->U[0], b->U[1],
                     b->G[0], b->G[1], b->net_start, b->pin_cell, b->cps, b->cpins, b->pin_off,
-                    b->pin_grad, b->map_field, b->map_mov, b->map_fix, b->ex, b->ey, b->tA, b->tB,
-                    b->map_mean, b->map_max, b->order, b->sort_count, b->sort_off, b->dens_tab, b->pre_cell,
+                    b->pin_grad, b->map_field, b->map_mov, b->map_fix, b->acc_field, b->acc_mov, b->ex, b->ey, b->tA, b->tB,
+                    b->map_mean, b->map_max, b->sorted, b->sort_count, b->sort_off, b->dens_tab, b->pre_cell,
                     b->pre_sort, b->pre_bin, b->pre_wa, b->pre_dens[0], b->pre_dens[1],
                     b->pre_dens[2], b->buckets, b->part_wa, b->part_ovf, b->part_cell};
     for (void *p : ptrs)
@@ -288,6 +292,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
         I.rho_t = p->target_density;
         I.cap = p->target_density * L.bw * L.bh;
         I.mov_area = L.mov_area;
+        I.dens_unit = (double)L.bw * (double)L.bh / kDensFixUnits;
         I.hpwl_ref = L.hpwl_ref;
         I.init_disp = (double)p->init_disp_bins * L.bw;
         I.gamma_base = (double)p->gamma_bins * L.bw;
@@ -390,6 +395,8 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     EPB_TRY(dalloc(&b->cpins, cpins.size()));
     EPB_TRY(dalloc(&b->map_field, nb));
     EPB_TRY(dalloc(&b->map_mov, nb));
+    EPB_TRY(dalloc(&b->acc_field, nb));
+    EPB_TRY(dalloc(&b->acc_mov, nb));
     EPB_TRY(dalloc(&b->map_fix, nb));
     EPB_TRY(dalloc(&b->ex, nb));
     EPB_TRY(dalloc(&b->ey, nb));
@@ -397,7 +404,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     EPB_TRY(dalloc(&b->tB, 2 * nb));
     EPB_TRY(dalloc(&b->map_mean, n));
     EPB_TRY(dalloc(&b->map_max, n));
-    EPB_TRY(dalloc(&b->order, nc));
+    EPB_TRY(dalloc(&b->sorted, nc));
     EPB_TRY(dalloc(&b->sort_count, (size_t)n * kSortTiles));
     EPB_TRY(dalloc(&b->sort_off, (size_t)n * kSortTiles));
     EPB_TRY(dalloc(&b->dens_tab, (size_t)n * (kSortTiles + 1)));

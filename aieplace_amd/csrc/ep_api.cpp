@@ -92,7 +92,7 @@ void enqueue_eval(Ctx &c, const float2 *pos, const float2 *vref, const float2 *g
 {
     launch_fill_maps(c, s);
     launch_sort(c, pos, s);
-    launch_density(c, pos, s);
+    launch_density(c, s);
     launch_overflow(c, s);
     launch_wa(c, pos, s);
     launch_field(c, c.prm.precision, s);
@@ -225,8 +225,8 @@ static void destroy_ctx(ep_ctx *c)
         if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
     void *ptrs[] = {c->wh, c->net_start, c->pin_cell, c->pin_off, c->pin_grad, c->cell_pin_start,
                     c->cell_pins, c->V[0], c->V[1], c->U[0], c->U[1], c->G[0], c->G[1],
-                    c->map_field, c->map_mov, c->map_fix, c->ex, c->ey, c->tA, c->tB, c->part,
-                    c->order, c->sort_count, c->sort_off, c->dens_tab,
+                    c->map_field, c->map_mov, c->map_fix, c->acc_field, c->acc_mov, c->ex, c->ey, c->tA, c->tB, c->part,
+                    c->sorted, c->sort_count, c->sort_off, c->dens_tab,
                     c->d_scal, c->d_res, c->d_mlp_w};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -528,8 +528,10 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     }
     EP_TRY(dalloc(&c->map_field, nb));
     EP_TRY(dalloc(&c->map_mov, nb));
+    EP_TRY(dalloc(&c->acc_field, nb));
+    EP_TRY(dalloc(&c->acc_mov, nb));
     EP_TRY(dalloc(&c->map_fix, nb));
-    EP_TRY(dalloc(&c->order, std::max(1, c->n_place)));
+    EP_TRY(dalloc(&c->sorted, std::max(1, c->n_place)));
     EP_TRY(dalloc(&c->sort_count, kSortTiles));
     EP_TRY(dalloc(&c->sort_off, kSortTiles));
     EP_TRY(dalloc(&c->dens_tab, kSortTiles + 1));
@@ -669,7 +671,7 @@ static ep_status run_density(ep_ctx *c)
 {
     launch_fill_maps(*c, c->stream);
     launch_sort(*c, c->V[c->cur], c->stream);
-    launch_density(*c, c->V[c->cur], c->stream);
+    launch_density(*c, c->stream);
     launch_overflow(*c, c->stream);
     launch_finalize(*c, c->stream);
     (void)hipMemcpyAsync(c->h_res, c->d_res, sizeof(DevResult), hipMemcpyDeviceToHost, c->stream);
@@ -1074,7 +1076,8 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
     // field input must be a real map: build it once at the current positions
     launch_fill_maps(*c, c->stream);
     launch_sort(*c, c->V[a], c->stream);
-    launch_density(*c, c->V[a], c->stream);
+    launch_density(*c, c->stream);
+    launch_overflow(*c, c->stream);
     hipEvent_t e0, e1;
     EP_HIP(hipEventCreate(&e0));
     EP_HIP(hipEventCreate(&e1));
@@ -1084,17 +1087,25 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
         double bytes, flops;
     };
     const double M = c->Mx;
-    const double gemm_flops = 12.0 * M * M * M * (c->prm.precision == EP_PREC_FP32 ? 1.0 : 3.0);
+    // symmetric transform products: 3 Mx My (Mx + My) flops per solve (6 M^3
+    // square, half the dense 12 M^3); the split forms run three products each
+    const double Mxd = c->Mx, Myd = c->My;
+    const double gemm_flops = 3.0 * Mxd * Myd * (Mxd + Myd) * (c->prm.precision == EP_PREC_FP32 ? 1.0 : 3.0);
+    // reads 6 maps, writes 7; the halves of the six matrices: 1.5 (Mx^2 + My^2) floats
+    const double field_bytes = 4.0 * 13 * nb + 4.0 * 1.5 * (Mxd * Mxd + Myd * Myd);
     Item items[] = {
-        {"cell_sort", 3, 2 * 16.0 * np + 8.0 * np + 8.0 * kSortTiles, 0},
-        {"density_scatter", 1, 4.0 * np + 16.0 * np + 8.0 * nb, 0},
-        {"overflow", 1, 8.0 * nb, 0},
+        // positions and sizes read twice, packed cells written once
+        {"cell_sort", 3, 2 * 16.0 * np + 16.0 * np + 8.0 * kSortTiles, 0},
+        // packed cells read; two 64-bit accumulator maps flushed
+        {"density_scatter", 1, 16.0 * np + 16.0 * nb, 0},
+        // two accumulator maps and the fixed map read, two FP32 maps written
+        {"overflow", 1, 28.0 * nb, 0},
         {"wa_grad", 1, 4.0 * (N + 1) + 12.0 * P + 8.0 * c->n_all + 8.0 * P, 0},
-        {"field_transforms", 4, 4.0 * 6 * nb + 4.0 * 6 * M * M, gemm_flops},
+        {"field_transforms", 4, field_bytes, gemm_flops},
         {"cell_grad", 1, 36.0 * np + 12.0 * pm + 4.0 * c->n_mov + 8.0 * nb, 0},
         {"nesterov_trial", 1, 40.0 * np, 0},
         {"finalize", 1, 8.0 * c->part_total, 0},
-        {"fill_maps", 1, 12.0 * nb, 0},
+        {"fill_maps", 1, 16.0 * nb, 0},
     };
     int k = 0;
     for (const Item &it : items) {
@@ -1102,7 +1113,7 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
         for (int r = 0; r < reps; ++r) {
             const std::string nm = it.name;
             if (nm == "cell_sort") launch_sort(*c, c->V[a], c->stream);
-            else if (nm == "density_scatter") launch_density(*c, c->V[a], c->stream);
+            else if (nm == "density_scatter") launch_density(*c, c->stream);
             else if (nm == "overflow") launch_overflow(*c, c->stream);
             else if (nm == "wa_grad") launch_wa(*c, c->V[a], c->stream);
             else if (nm == "field_transforms") launch_field(*c, c->prm.precision, c->stream);
@@ -1133,7 +1144,8 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
     // leave the maps consistent with the current positions
     launch_fill_maps(*c, c->stream);
     launch_sort(*c, c->V[a], c->stream);
-    launch_density(*c, c->V[a], c->stream);
+    launch_density(*c, c->stream);
+    launch_overflow(*c, c->stream);
     return sync(*c);
 }
 

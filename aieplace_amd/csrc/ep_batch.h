@@ -1,7 +1,7 @@
 // ep_batch.h -- device data of the batched engine (include/eplace_batch.h).
 //
 // All instances share concatenated device arrays.  Instance i owns
-//   cells [cell_off, cell_off + n_all)   of wh, V[2], U[2], G[2], order
+//   cells [cell_off, cell_off + n_all)   of wh, V[2], U[2], G[2], sorted
 //   bins  [map_off, map_off + M*M)       of map_field/map_mov/map_fix/ex/ey
 //   bins  [2*map_off, 2*map_off + 2*M*M) of the transform scratch tA/tB
 //   nets and pins: contiguous ranges of net_start/pin_cell/pin_off/pin_grad,
@@ -31,6 +31,7 @@ struct BInst {
     float xl, yl, xh, yh, bw, bh, sqrt2_bw, sqrt2_bh;
     float rho_t, cap;      // target density, target density * bin area
     double mov_area;
+    double dens_unit;      // bin area / 2^32: value of one fixed-point density unit
     double hpwl_ref;       // mu-schedule reference
     double init_disp;      // probe displacement (init_disp_bins * bw)
     double gamma_base;     // gamma_bins * bw
@@ -82,11 +83,13 @@ struct BDev {
     float2 *pin_grad;
     const int32_t *cps, *cpins;  // global cell -> pin CSR (empty for non-movable cells)
     float *map_field, *map_mov, *map_fix, *ex, *ey;
+    unsigned long long *acc_field, *acc_mov;  // fixed-point scatter targets (2^32 per bin area)
     float *map_mean;             // [n_inst]
     uint32_t *map_max;           // [n_inst] float bits of max |map - mean|
-    int32_t *order, *sort_count, *sort_off;
+    float4 *sorted;              // packed cells in coarse-tile order (per instance at cell_off)
+    int32_t *sort_count, *sort_off;
     int32_t *dens_tab;           // [n_inst * (kSortTiles + 1)] density block tables
-    int32_t dens_wcap;           // LDS window capacity per map (bins), max over instances
+    int32_t dens_wcap;           // LDS window capacity per plane (bins), max over instances
     const int32_t *pre_cell;     // 256 cells per block
     const int32_t *pre_sort;     // kBlock * kSortCellsPerThread cells per block
     const int32_t *pre_dens;     // density blocks per instance (dens_grid bound)

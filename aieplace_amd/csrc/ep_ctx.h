@@ -12,6 +12,8 @@
 //   pin_grad[P]          float2  per-pin WA partials (summed per cell by CSR)
 //   cell_pin_start[n_mov+1], cell_pins[P_mov]   cell -> pin CSR
 //   map_field, map_mov, map_fix [Mx*My] float  density maps (x-major)
+//   acc_field, acc_mov [Mx*My] uint64  scatter accumulators (2^32 per bin area)
+//   sorted[n_place]      float4  cells (x, y, +-w, h) in coarse-tile order
 //   ex, ey [Mx*My]        float  field; spectral / transform scratch
 //   transform matrices (FP32 and split BF16/FP16 copies)
 #pragma once
@@ -47,6 +49,11 @@ struct DensPlan {
 };
 
 constexpr int kDensMaxWin = 6400;  // LDS bins per map of the largest window
+// Fixed-point density units per bin area (the maps accumulate in uint64).
+constexpr double kDensFixUnits = 4294967296.0;  // 2^32
+
+// LDS bytes of a density window plan: two planes of 64-bit accumulators.
+inline size_t dens_lds_bytes(const DensPlan &p) { return 2 * sizeof(unsigned long long) * (size_t)p.wcap; }
 
 // Plan for an Mx x My grid whose smoothed cells are at most max_w x max_h wide.
 inline DensPlan make_dens_plan(int Mx, int My, double bw, double bh, double max_w, double max_h,
@@ -117,7 +124,8 @@ struct Ctx;
 // ---- kernel launchers (defined in the .hip files) ----
 void launch_fill_maps(Ctx &c, hipStream_t s);
 void launch_sort(Ctx &c, const float2 *pos, hipStream_t s);
-void launch_density(Ctx &c, const float2 *pos, hipStream_t s);
+// Scatters the cells packed by the last launch_sort (same positions).
+void launch_density(Ctx &c, hipStream_t s);
 void launch_overflow(Ctx &c, hipStream_t s);
 void launch_wa(Ctx &c, const float2 *pos, hipStream_t s);
 void launch_field(Ctx &c, int precision, hipStream_t s);
@@ -164,7 +172,8 @@ struct Ctx {
 
     // density + field
     float *map_field = nullptr, *map_mov = nullptr, *map_fix = nullptr;
-    int32_t *order = nullptr;                        // cells in spatial (coarse tile) order
+    unsigned long long *acc_field = nullptr, *acc_mov = nullptr;  // fixed-point scatter targets
+    float4 *sorted = nullptr;                        // packed cells in coarse-tile order
     int32_t *sort_count = nullptr, *sort_off = nullptr;  // kSortTiles each
     int32_t *dens_tab = nullptr;                     // kSortTiles + 1: density block table
     DensPlan dens{};

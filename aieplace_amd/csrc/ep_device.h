@@ -377,81 +377,101 @@ __device__ inline void super_window(int s, int level, int Mx, int My, int ext, i
     ny = min(My, by1 + ext) - y0;
 }
 
-// Adds v at bin (i, j): into the LDS window when inside, else straight into HBM.
-__device__ inline void dens_add(float *__restrict__ win, float *__restrict__ map, int i, int j,
-                                int x0, int y0, int nx, int ny, int My, bool lds, float v)
+// Packed sorted cell: (x, y, w, h) with w negated for movable cells (they also
+// enter the movable-area map).  Written by the sort's scatter pass.
+__device__ inline float4 pack_cell(float2 p, float2 s, bool movable)
 {
-    const unsigned li = (unsigned)(i - x0), lj = (unsigned)(j - y0);
-    if (lds && li < (unsigned)nx && lj < (unsigned)ny)
-        atomicAdd(&win[li * ny + lj], v);
-    else
-        unsafeAtomicAdd(&map[(size_t)i * My + j], v);
+    return make_float4(p.x, p.y, movable ? -s.x : s.x, s.y);
 }
 
-// Density scatter of cells order[c0, c1) (global cell ids; ids < mov_end are
-// movable and also enter map_mov) through the LDS window (x0, y0, nx, ny).
-// win holds 2 * wcap floats of dynamic LDS (field map, movable map).
-template <int CPT>
-__device__ inline void density_window(const int32_t *__restrict__ order, int c0, int c1, int mov_end,
-                                      const float2 *__restrict__ pos, const float2 *__restrict__ wh,
-                                      const GeomArgs &g, int Mx, int My, int x0, int y0, int nx,
-                                      int ny, float *__restrict__ map_field,
-                                      float *__restrict__ map_mov, float *win, int wcap)
+// The density maps accumulate in 64-bit fixed point, 2^32 units per bin area:
+// integer adds are associative, so the maps (and everything computed from
+// them) do not depend on the order in which blocks and lanes add, and LDS
+// integer atomics run several times faster than LDS float atomics on CDNA.
+// A contribution is at most one bin area, so a bin overflows only past 2^31
+// bin areas of overlap.
+constexpr float kDensFix = (float)kDensFixUnits;
+
+__device__ inline unsigned long long dens_fix(float v) { return (unsigned long long)(v * kDensFix); }
+
+// Adds the overlaps of rectangle r (in bin areas, scaled by sc) to plane
+// `plane` of the LDS window (x0, y0, nx, ny) when r lies inside it, else
+// straight into `acc`.  The window is the dynamic LDS block, declared here and
+// not passed as a pointer so the adds compile to LDS atomics.  Bins of
+// [i0, i1] x [j0, j1] never have a negative overlap, so no test is needed per
+// bin; the y overlaps are shared by every row of the rectangle.
+__device__ inline void scatter_rect(const Rect &r, float sc, int plane,
+                                    unsigned long long *__restrict__ acc, int x0, int y0, int nx,
+                                    int ny, int My, bool lds, int wcap)
 {
+    extern __shared__ unsigned long long dens_win[];
+    if (r.i0 > r.i1 || r.j0 > r.j1) return;
+    const int dj = r.j1 - r.j0;
+    if (lds && dj < 4 && r.i0 >= x0 && r.i1 < x0 + nx && r.j0 >= y0 && r.j1 < y0 + ny) {
+        float oy[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) oy[q] = q <= dj ? ovl(r.ty0, r.ty1, r.j0 + q) * sc : 0.0f;
+        int w = plane * wcap + (r.i0 - x0) * ny + (r.j0 - y0);
+        for (int i = r.i0; i <= r.i1; ++i, w += ny) {
+            const float ox = ovl(r.tx0, r.tx1, i);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (q <= dj) atomicAdd(&dens_win[w + q], dens_fix(ox * oy[q]));
+        }
+        return;
+    }
+    for (int i = r.i0; i <= r.i1; ++i) {
+        const float ox = ovl(r.tx0, r.tx1, i);
+        for (int j = r.j0; j <= r.j1; ++j) {
+            const unsigned long long v = dens_fix(ox * ovl(r.ty0, r.ty1, j) * sc);
+            const unsigned li = (unsigned)(i - x0), lj = (unsigned)(j - y0);
+            if (lds && li < (unsigned)nx && lj < (unsigned)ny)
+                atomicAdd(&dens_win[plane * wcap + li * ny + lj], v);
+            else
+                atomicAdd(&acc[(size_t)i * My + j], v);
+        }
+    }
+}
+
+// Density scatter of the sorted cells [c0, c1) through the LDS window
+// (x0, y0, nx, ny): 2 * wcap 64-bit accumulators of dynamic LDS (field plane:
+// smoothed rectangles scaled by their area ratio; movable plane: exact
+// rectangles of movable cells), flushed into acc_field / acc_mov with one
+// global atomic per non-zero bin.
+template <int CPT>
+__device__ inline void density_window(const float4 *__restrict__ sorted, int c0, int c1,
+                                      const GeomArgs &g, int Mx, int My, int x0, int y0, int nx,
+                                      int ny, unsigned long long *__restrict__ acc_field,
+                                      unsigned long long *__restrict__ acc_mov, int wcap)
+{
+    extern __shared__ unsigned long long dens_win[];
     const int nw = nx * ny;
     const bool lds = nw <= wcap;
-    float *win0 = win, *win1 = win + wcap;
     if (lds) {
         for (int t = threadIdx.x; t < nw; t += kBlock) {
-            win0[t] = 0.0f;
-            win1[t] = 0.0f;
+            dens_win[t] = 0ull;
+            dens_win[wcap + t] = 0ull;
         }
         __syncthreads();
     }
-    const float area_scale = g.v[4] * g.v[5];
-    // all loads of the thread's cells first, so they are in flight together
-    int cid[CPT];
-    float2 cp[CPT], cs[CPT];
+    float4 cell[CPT];  // loads first, so they are in flight together
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int idx = c0 + k * kBlock + threadIdx.x;
-        cid[k] = idx < c1 ? order[idx] : -1;
+        cell[k] = idx < c1 ? sorted[idx] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
-    for (int k = 0; k < CPT; ++k)
-        if (cid[k] >= 0) {
-            cp[k] = pos[cid[k]];
-            cs[k] = wh[cid[k]];
-        }
-#pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        if (cid[k] < 0) break;
-        const int c = cid[k];
-        const float2 p = cp[k], s = cs[k];
-        Rect rs;
-        stretched_rect(p, s, g, Mx, My, rs);
-        const float sc = rs.ratio * area_scale;
-        for (int i = rs.i0; i <= rs.i1; ++i) {
-            const float ox = ovl(rs.tx0, rs.tx1, i);
-            if (ox <= 0.0f) continue;
-            for (int j = rs.j0; j <= rs.j1; ++j) {
-                const float oy = ovl(rs.ty0, rs.ty1, j);
-                if (oy <= 0.0f) continue;
-                dens_add(win0, map_field, i, j, x0, y0, nx, ny, My, lds, ox * oy * sc);
-            }
-        }
-        if (c < mov_end) {
-            Rect re;
-            exact_rect(p, s, g, Mx, My, re);
-            for (int i = re.i0; i <= re.i1; ++i) {
-                const float ox = ovl(re.tx0, re.tx1, i);
-                if (ox <= 0.0f) continue;
-                for (int j = re.j0; j <= re.j1; ++j) {
-                    const float oy = ovl(re.ty0, re.ty1, j);
-                    if (oy <= 0.0f) continue;
-                    dens_add(win1, map_mov, i, j, x0, y0, nx, ny, My, lds, ox * oy * area_scale);
-                }
-            }
+        if (c0 + k * kBlock + (int)threadIdx.x >= c1) break;
+        const bool movable = cell[k].z < 0.0f;
+        const float2 p = make_float2(cell[k].x, cell[k].y);
+        const float2 s = make_float2(fabsf(cell[k].z), cell[k].w);
+        Rect r;
+        stretched_rect(p, s, g, Mx, My, r);
+        scatter_rect(r, r.ratio, 0, acc_field, x0, y0, nx, ny, My, lds, wcap);
+        if (movable) {
+            exact_rect(p, s, g, Mx, My, r);
+            scatter_rect(r, 1.0f, 1, acc_mov, x0, y0, nx, ny, My, lds, wcap);
         }
     }
     if (lds) {
@@ -459,9 +479,9 @@ __device__ inline void density_window(const int32_t *__restrict__ order, int c0,
         for (int t = threadIdx.x; t < nw; t += kBlock) {
             const int i = x0 + t / ny, j = y0 + t % ny;
             const size_t b = (size_t)i * My + j;
-            const float a = win0[t], m = win1[t];
-            if (a != 0.0f) unsafeAtomicAdd(&map_field[b], a);
-            if (m != 0.0f) unsafeAtomicAdd(&map_mov[b], m);
+            const unsigned long long a = dens_win[t], m = dens_win[wcap + t];
+            if (a) atomicAdd(&acc_field[b], a);
+            if (m) atomicAdd(&acc_mov[b], m);
         }
     }
 }

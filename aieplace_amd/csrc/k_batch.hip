@@ -90,6 +90,7 @@ __global__ __launch_bounds__(kBlock) void kb_move(BDev d, int mode)
 
 // ------------------------------------------------------------------ maps
 
+// Clears the instance's fixed-point scatter accumulators.
 __global__ __launch_bounds__(kBlock) void kb_fill(BDev d)
 {
     const int i = find_inst(d.pre_bin, d.n_inst, blockIdx.x);
@@ -102,8 +103,8 @@ __global__ __launch_bounds__(kBlock) void kb_fill(BDev d)
         const int b = base + k * kBlock + threadIdx.x;
         if (b < nb) {
             const size_t o = (size_t)I.map_off + b;
-            d.map_field[o] = I.rho_t * d.map_fix[o];
-            d.map_mov[o] = 0.0f;
+            d.acc_field[o] = 0ull;
+            d.acc_mov[o] = 0ull;
         }
     }
 }
@@ -158,14 +159,17 @@ __global__ __launch_bounds__(kBlock) void kb_sort_scatter(BDev d, int at)
     __syncthreads();
     const int base = (blockIdx.x - d.pre_sort[i]) * kBlock * kSortCellsPerThread;
     int key[kSortCellsPerThread], rank[kSortCellsPerThread];
+    float4 cell[kSortCellsPerThread];
 #pragma unroll
     for (int k = 0; k < kSortCellsPerThread; ++k) {
         const int idx = base + k * kBlock + threadIdx.x;
         key[k] = -1;
         if (idx < I.n_place) {
             const int c = I.cell_off + idx;
-            key[k] = cell_key(pos[c], d.wh[c], g, I.M, I.M);
+            const float2 p = pos[c], s = d.wh[c];
+            key[k] = cell_key(p, s, g, I.M, I.M);
             rank[k] = atomicAdd(&hist[key[k]], 1);
+            cell[k] = pack_cell(p, s, idx < I.n_mov);
         }
     }
     __syncthreads();
@@ -173,18 +177,17 @@ __global__ __launch_bounds__(kBlock) void kb_sort_scatter(BDev d, int at)
     for (int t = threadIdx.x; t < kSortTiles; t += kBlock)
         if (hist[t]) base_of[t] = atomicAdd(&off[t], hist[t]);
     __syncthreads();
-    int32_t *order = d.order + I.cell_off;
+    float4 *sorted = d.sorted + I.cell_off;
 #pragma unroll
     for (int k = 0; k < kSortCellsPerThread; ++k)
-        if (key[k] >= 0) order[base_of[key[k]] + rank[k]] = I.cell_off + base + k * kBlock + threadIdx.x;
+        if (key[k] >= 0) sorted[base_of[key[k]] + rank[k]] = cell[k];
 }
 
 // Density scatter (as k_density): the instance's blocks take up to kBlock * CPT
-// cells of one super tile each and accumulate them in an LDS window.
+// packed cells of one super tile each and accumulate them in an LDS window.
 template <int CPT>
-__global__ __launch_bounds__(kBlock) void kb_density(BDev d, int at)
+__global__ __launch_bounds__(kBlock) void kb_density(BDev d)
 {
-    extern __shared__ float win[];
     const int i = find_inst(d.pre_dens, d.n_inst, blockIdx.x);
     const BState *S = d.st + i;
     if (S->done) return;
@@ -198,12 +201,12 @@ __global__ __launch_bounds__(kBlock) void kb_density(BDev d, int at)
     int c0, c1, x0, y0, nx, ny;
     dens_cell_range(d.sort_off + (size_t)i * kSortTiles, s, level, b - dtab[s], kBlock * CPT, c0, c1);
     super_window(s, level, I.M, I.M, I.dens_ext, x0, y0, nx, ny);
-    density_window<CPT>(d.order + I.cell_off, c0, c1, I.cell_off + I.n_mov, eval_pos(d, S->cur, at),
-                        d.wh, inst_geom(I), I.M, I.M, x0, y0, nx, ny, d.map_field + I.map_off,
-                        d.map_mov + I.map_off, win, d.dens_wcap);
+    density_window<CPT>(d.sorted + I.cell_off, c0, c1, inst_geom(I), I.M, I.M, x0, y0, nx, ny,
+                        d.acc_field + I.map_off, d.acc_mov + I.map_off, d.dens_wcap);
 }
 
-// Overflow numerator and field-map sum/max/-min per block (as k_overflow).
+// Accumulators -> FP32 maps, overflow numerator and field-map sum/max/-min per
+// block (as k_overflow).
 __global__ __launch_bounds__(kBlock) void kb_overflow(BDev d)
 {
     const int i = find_inst(d.pre_bin, d.n_inst, blockIdx.x);
@@ -217,9 +220,14 @@ __global__ __launch_bounds__(kBlock) void kb_overflow(BDev d)
         const int b = base + k * kBlock + threadIdx.x;
         if (b < nb) {
             const size_t o = (size_t)I.map_off + b;
-            const float v = d.map_mov[o] + I.rho_t * d.map_fix[o] - I.cap;
+            const float fx = d.map_fix[o];
+            const float mv = (float)((double)d.acc_mov[o] * I.dens_unit);
+            const float fd = I.rho_t * fx + (float)((double)d.acc_field[o] * I.dens_unit);
+            d.map_mov[o] = mv;
+            d.map_field[o] = fd;
+            const float v = mv + I.rho_t * fx - I.cap;
             if (v > 0.0f) acc[0] += (double)v;
-            const double f = d.map_field[o];
+            const double f = fd;
             acc[1] += f;
             acc[2] = fmax(acc[2], f);
             acc[3] = fmax(acc[3], -f);
@@ -526,13 +534,13 @@ void bl_maps(const BDev &d, int at, int bin_blocks, int sort_blocks, int dens_bl
     hipLaunchKernelGGL(kb_sort_count, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
     hipLaunchKernelGGL(kb_sort_scan, dim3(d.n_inst), dim3(kSortScanThreads), 0, s, d, kBlock * dens_cpt);
     hipLaunchKernelGGL(kb_sort_scatter, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
-    const size_t lds = 2 * sizeof(float) * d.dens_wcap;
+    const size_t lds = 2 * sizeof(unsigned long long) * (size_t)d.dens_wcap;
     if (dens_cpt == 2)
-        hipLaunchKernelGGL(kb_density<2>, dim3(dens_blocks), dim3(kBlock), lds, s, d, at);
+        hipLaunchKernelGGL(kb_density<2>, dim3(dens_blocks), dim3(kBlock), lds, s, d);
     else if (dens_cpt == 8)
-        hipLaunchKernelGGL(kb_density<8>, dim3(dens_blocks), dim3(kBlock), lds, s, d, at);
+        hipLaunchKernelGGL(kb_density<8>, dim3(dens_blocks), dim3(kBlock), lds, s, d);
     else
-        hipLaunchKernelGGL(kb_density<4>, dim3(dens_blocks), dim3(kBlock), lds, s, d, at);
+        hipLaunchKernelGGL(kb_density<4>, dim3(dens_blocks), dim3(kBlock), lds, s, d);
     hipLaunchKernelGGL(kb_overflow, dim3(bin_blocks), dim3(kBlock), 0, s, d);
     hipLaunchKernelGGL(kb_map_stats, dim3(d.n_inst), dim3(kBlock), 0, s, d);
 }

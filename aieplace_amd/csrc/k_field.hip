@@ -46,13 +46,16 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int BM = 64, BN = 64;
 constexpr float kF16Top = 14.0f;  // scaled data max lies in [2^14, 2^15)
 
 struct GemmArgs {
-    const float *A[2];          // FP32 constant operand [M][K]
-    const uint16_t *Ah[2];      // split constant operand, hi plane (BF16 or FP16 bits)
-    const uint16_t *Al[2];      //                          lo plane
+    const float *A[2];          // FP32 even half of the constant operand [M/2][K/2]
+    const float *Ao[2];         //      odd half
+    const uint16_t *Ah[2];      // split even half, hi plane (BF16 or FP16 bits)
+    const uint16_t *Al[2];      //                  lo plane
+    const uint16_t *Aoh[2];     // split odd half, hi plane
+    const uint16_t *Aol[2];     //                 lo plane
+    float sgn[2];               // inverse products: +1 cosine, -1 sine
     const float *B[2];          // data operand [N][K], FP32
     float *C[2];                // [M][N]
     float *C2[2];               // second output of the spectral epilogue
@@ -105,68 +108,146 @@ __device__ inline void track_max(uint32_t *slot, float m)
     }
 }
 
+// ---------------------------------------------------------------- symmetric products
+//
+// Every transform matrix is even or odd about its centre:
+//   forward  T[f][M-1-x]  = (-1)^f T[f][x]                 (cosine rows)
+//   inverse  T'[M-1-x][f] = s (-1)^f T'[x][f],  s = +1 cosine, -1 sine
+// so each M x K product splits into two (M/2) x (K/2) products over the even
+// and odd halves E and O (half the flops):
+//   forward  B+[n][x] = b[x] + b[K-1-x],  B-[n][x] = b[x] - b[K-1-x]   (x < K/2)
+//            C[2r] = (Ae B+)[r],  C[2r+1] = (Ao B-)[r]
+//   inverse  Be[n][f] = b[2f],  Bo[n][f] = b[2f+1]
+//            E = Ae Be, O = Ao Bo;  C[x] = E + O,  C[M-1-x] = s (E - O)   (x < M/2)
+// Ae/Ao are the even/odd rows (forward) or columns (inverse) of the left half
+// of T, stored (M/2) x (K/2) row-major.  The forward butterfly runs while the
+// data operand is staged into LDS; the inverse one in the epilogue.
+//
+// Block tile: 64 half-rows x 32 columns of both E and O.  Waves 0-1 hold E
+// (half-rows 0-31 and 32-63), waves 2-3 hold O; each wave one 32x32 MFMA
+// accumulator.
+constexpr int SBM = 64, SBN = 32;
+
+// Stores a wave's accumulator (times mul) through the symmetric epilogue;
+// returns the largest |value| written.  xch: >= SBM * SBN floats of LDS that
+// the caller no longer reads.
+template <bool INV, int EPI>
+__device__ inline float sym_store(const GemmArgs &a, int z, int h0, int n0, const f32x16 &acc,
+                                  float mul, float *xch)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int eo = wv >> 1, wm = (wv & 1) * 32;
+    const int Mh = a.M >> 1, col = n0 + (lane & 31);
+    float m = 0.0f;
+    if (!INV) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = h0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+            if (r < Mh && col < a.N) m = fmaxf(m, store_out(a, z, EPI, 2 * r + eo, col, acc[i] * mul));
+        }
+        return m;
+    }
+    __syncthreads();  // every wave is done with the staged operands
+    if (eo) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            xch[(wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)) * SBN + (lane & 31)] = acc[i];
+    }
+    __syncthreads();
+    if (!eo) {
+        const float sg = a.sgn[z];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int rl = wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), r = h0 + rl;
+            const float e = acc[i], o = xch[rl * SBN + (lane & 31)];
+            if (r < Mh && col < a.N) {
+                m = fmaxf(m, store_out(a, z, 0, r, col, (e + o) * mul));
+                m = fmaxf(m, store_out(a, z, 0, a.M - 1 - r, col, sg * (e - o) * mul));
+            }
+        }
+    }
+    return m;
+}
+
 // ---------------------------------------------------------------- FP32
 
-constexpr int BK32 = 16;
+constexpr int SK32 = 16;  // half-k per step
 
-// 64x64 block tile, 4 waves of 32x32; global tiles [64][16] of A and Bt are
-// staged transposed into LDS ([k][m], [k][n]) for the 32x32x2 operand layout.
-template <int EPI>
-__device__ inline void gemm_f32_tile(const GemmArgs &a, int z, int m0, int n0)
+template <bool INV, int EPI>
+__device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
 {
-    const float *__restrict__ A = a.A[z];
+    const int N = a.N, K = a.K, Mh = a.M >> 1, Kh = K >> 1;
+    const float *__restrict__ Ae = a.A[z];
+    const float *__restrict__ Ao = a.Ao[z];
     const float *__restrict__ B = a.B[z];
-    const int M = a.M, N = a.N, K = a.K;
-    __shared__ float As[BK32][BM + 4];
-    __shared__ float Bs[BK32][BN + 4];
+    __shared__ float sA[2][SK32][SBM + 4];
+    __shared__ float sB[2][SK32][SBN + 4];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
-    const int lr = t >> 2, lk = (t & 3) * 4;
-    const bool a_ok = (m0 + lr) < M, b_ok = (n0 + lr) < N;
+    const int eo = wv >> 1, wm = (wv & 1) * 32;
+    const int ar = t >> 2, ak = (t & 3) * 4;  // A: 64 rows x 16 k, 4 per thread and half
+    const int br = t >> 3, bk = (t & 7) * 2;  // B: 32 rows x 16 half-k, 2 per thread and plane
+    const bool a_ok = h0 + ar < Mh, b_ok = n0 + br < N;
+    const float bias = (!INV && a.bbias && b_ok) ? *a.bbias : 0.0f;
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-
-    const float bias = (a.bbias && b_ok) ? *a.bbias : 0.0f;
-    float4 av = make_float4(0, 0, 0, 0), bv = make_float4(0, 0, 0, 0);
-    if (a_ok) av = *reinterpret_cast<const float4 *>(&A[(size_t)(m0 + lr) * K + lk]);
-    if (b_ok) bv = *reinterpret_cast<const float4 *>(&B[(size_t)(n0 + lr) * K + lk]);
-    for (int k0 = 0; k0 < K; k0 += BK32) {
-        __syncthreads();
-        As[lk + 0][lr] = av.x; As[lk + 1][lr] = av.y; As[lk + 2][lr] = av.z; As[lk + 3][lr] = av.w;
-        Bs[lk + 0][lr] = bv.x - bias; Bs[lk + 1][lr] = bv.y - bias;
-        Bs[lk + 2][lr] = bv.z - bias; Bs[lk + 3][lr] = bv.w - bias;
-        __syncthreads();
-        const int kn = k0 + BK32;
-        if (kn < K) {  // prefetch the next tiles while the MFMAs run
-            if (a_ok) av = *reinterpret_cast<const float4 *>(&A[(size_t)(m0 + lr) * K + kn + lk]);
-            if (b_ok) bv = *reinterpret_cast<const float4 *>(&B[(size_t)(n0 + lr) * K + kn + lk]);
+    float4 ae, ao;
+    float2 b0, b1;
+    auto load = [&](int k0) {
+        ae = ao = make_float4(0, 0, 0, 0);
+        b0 = b1 = make_float2(0, 0);
+        if (a_ok && k0 + ak < Kh) {
+            ae = *reinterpret_cast<const float4 *>(&Ae[(size_t)(h0 + ar) * Kh + k0 + ak]);
+            ao = *reinterpret_cast<const float4 *>(&Ao[(size_t)(h0 + ar) * Kh + k0 + ak]);
         }
+        if (b_ok && k0 + bk < Kh) {
+            const float *row = B + (size_t)(n0 + br) * K;
+            if (INV) {
+                const float4 v = *reinterpret_cast<const float4 *>(&row[2 * (k0 + bk)]);
+                b0 = make_float2(v.x, v.z);  // b[2k], b[2k+2]
+                b1 = make_float2(v.y, v.w);  // b[2k+1], b[2k+3]
+            } else {
+                b0 = *reinterpret_cast<const float2 *>(&row[k0 + bk]);          // b[k], b[k+1]
+                b1 = *reinterpret_cast<const float2 *>(&row[K - 2 - k0 - bk]);  // b[K-2-k], b[K-1-k]
+            }
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < Kh; k0 += SK32) {
+        __syncthreads();
+        sA[0][ak + 0][ar] = ae.x; sA[0][ak + 1][ar] = ae.y; sA[0][ak + 2][ar] = ae.z; sA[0][ak + 3][ar] = ae.w;
+        sA[1][ak + 0][ar] = ao.x; sA[1][ak + 1][ar] = ao.y; sA[1][ak + 2][ar] = ao.z; sA[1][ak + 3][ar] = ao.w;
+        if (INV) {
+            sB[0][bk][br] = b0.x; sB[0][bk + 1][br] = b0.y;
+            sB[1][bk][br] = b1.x; sB[1][bk + 1][br] = b1.y;
+        } else {
+            const float d0 = b0.x - bias, d1 = b0.y - bias;  // b[k], b[k+1]
+            const float m0 = b1.y - bias, m1 = b1.x - bias;  // b[K-1-k], b[K-2-k]
+            sB[0][bk][br] = d0 + m0; sB[0][bk + 1][br] = d1 + m1;
+            sB[1][bk][br] = d0 - m0; sB[1][bk + 1][br] = d1 - m1;
+        }
+        __syncthreads();
+        if (k0 + SK32 < Kh) load(k0 + SK32);  // next tiles in flight during the MFMAs
 #pragma unroll
-        for (int kk = 0; kk < BK32; kk += 2) {
-            const float x = As[kk + (lane >> 5)][wm + (lane & 31)];
-            const float y = Bs[kk + (lane >> 5)][wn + (lane & 31)];
+        for (int kk = 0; kk < SK32; kk += 2) {
+            const float x = sA[eo][kk + (lane >> 5)][wm + (lane & 31)];
+            const float y = sB[eo][kk + (lane >> 5)][lane & 31];
             acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x, y, acc, 0, 0, 0);
         }
     }
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = m0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-        const int col = n0 + wn + (lane & 31);
-        if (row < M && col < N) store_out(a, z, EPI, row, col, acc[i]);
-    }
+    sym_store<INV, EPI>(a, z, h0, n0, acc, 1.0f, &sA[0][0][0]);
 }
 
-template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_f32(GemmArgs a)
+template <bool INV, int EPI>
+__global__ __launch_bounds__(256) void k_sym_f32(GemmArgs a)
 {
-    gemm_f32_tile<EPI>(a, blockIdx.z, blockIdx.y * BM, blockIdx.x * BN);
+    sym_f32_tile<INV, EPI>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
 // ---------------------------------------------------------------- split 16-bit
 
-constexpr int BK16 = 32;
-constexpr int LDK = BK16 + 8;  // padded LDS row (16-bit elements)
+constexpr int SK16 = 32;          // half-k per step (two 32x32x16 MFMA sub-steps)
+constexpr int SLDK = SK16 + 8;    // padded LDS row (16-bit elements)
 
 __device__ inline uint16_t bf16_rne(float x)
 {
@@ -200,115 +281,146 @@ __device__ inline f32x16 mfma16(const uint16_t *pa, const uint16_t *pb, f32x16 a
                                                   *reinterpret_cast<const bf16x8 *>(pb), acc, 0, 0, 0);
 }
 
-// 64x64 block tile, 4 waves of 32x32, K step 32 (two 32x32x16 MFMA sub-steps,
-// three products each).  LDS rows are [m or n][k] with k contiguous, so each
-// lane reads its 8 consecutive k values with one 16-byte load.
-template <int EPI, bool F16>
-__device__ inline void gemm_split_tile(const GemmArgs &a, int z, int m0, int n0)
+// Splits four values into hi and lo planes and stores them as 4 x 16 bits each.
+template <bool F16>
+__device__ inline void split_store4(const float (&v)[4], uint16_t *dh, uint16_t *dl)
 {
-    const uint16_t *__restrict__ Ah = a.Ah[z];
-    const uint16_t *__restrict__ Al = a.Al[z];
+    uint16_t h[4], l[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) split16<F16>(v[e], h[e], l[e]);
+    *reinterpret_cast<uint2 *>(dh) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+    *reinterpret_cast<uint2 *>(dl) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
+}
+
+template <bool INV, int EPI, bool F16>
+__device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
+{
+    const int N = a.N, K = a.K, Mh = a.M >> 1, Kh = K >> 1;
+    const uint16_t *__restrict__ Aeh = a.Ah[z];
+    const uint16_t *__restrict__ Ael = a.Al[z];
+    const uint16_t *__restrict__ Aoh = a.Aoh[z];
+    const uint16_t *__restrict__ Aol = a.Aol[z];
     const float *__restrict__ B = a.B[z];
-    const int M = a.M, N = a.N, K = a.K;
-    __shared__ __attribute__((aligned(16))) uint16_t sAh[BM][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sAl[BM][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sBh[BN][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sBl[BN][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sAh[2][SBM][SLDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sAl[2][SBM][SLDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sBh[2][SBN][SLDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sBl[2][SBN][SLDK];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
-    const int lr = t >> 2, lk = (t & 3) * 8;  // row and first k of this thread's 8-element chunk
-    const bool a_ok = (m0 + lr) < M, b_ok = (n0 + lr) < N;
-    // FP16: power-of-two scale of the data operand (exact; undone in the epilogue)
+    const int eo = wv >> 1, wm = (wv & 1) * 32;
+    const int ar = t >> 2, ak = (t & 3) * 8;  // A: 64 rows x 32 half-k, 8 per thread and plane
+    const int br = t >> 3, bk = (t & 7) * 4;  // B: 32 rows x 32 half-k, 4 per thread and plane
+    const bool a_ok = h0 + ar < Mh, b_ok = n0 + br < N;
+    // FP16: power-of-two scale of the data operand (exact; undone in the
+    // epilogue).  The forward butterfly adds two values: one binade lower.
     float bscale = 1.0f;
     if (F16) {
         const float bm = __uint_as_float(*a.bmax);
-        if (bm > 0.0f && isfinite(bm)) bscale = ldexpf(1.0f, (int)kF16Top - ilogbf(bm));
+        const int top = (int)kF16Top - (INV ? 0 : 1);
+        if (bm > 0.0f && isfinite(bm)) bscale = ldexpf(1.0f, top - ilogbf(bm));
     }
-    const float bias = (a.bbias && b_ok) ? *a.bbias : 0.0f;
+    const float bias = (!INV && a.bbias && b_ok) ? *a.bbias : 0.0f;
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-
-    uint4 ahv = make_uint4(0, 0, 0, 0), alv = make_uint4(0, 0, 0, 0);
-    float4 b0 = make_float4(0, 0, 0, 0), b1 = make_float4(0, 0, 0, 0);
+    uint4 aeh, ael, aoh, aol;
+    float4 b0, b1;
     auto load = [&](int k0) {
-        if (a_ok) {
-            ahv = *reinterpret_cast<const uint4 *>(&Ah[(size_t)(m0 + lr) * K + k0 + lk]);
-            alv = *reinterpret_cast<const uint4 *>(&Al[(size_t)(m0 + lr) * K + k0 + lk]);
+        aeh = ael = aoh = aol = make_uint4(0, 0, 0, 0);
+        b0 = b1 = make_float4(0, 0, 0, 0);
+        if (a_ok && k0 + ak < Kh) {
+            const size_t o = (size_t)(h0 + ar) * Kh + k0 + ak;
+            aeh = *reinterpret_cast<const uint4 *>(&Aeh[o]);
+            ael = *reinterpret_cast<const uint4 *>(&Ael[o]);
+            aoh = *reinterpret_cast<const uint4 *>(&Aoh[o]);
+            aol = *reinterpret_cast<const uint4 *>(&Aol[o]);
         }
-        if (b_ok) {
-            const float *src = &B[(size_t)(n0 + lr) * K + k0 + lk];
-            b0 = *reinterpret_cast<const float4 *>(src);
-            b1 = *reinterpret_cast<const float4 *>(src + 4);
+        if (b_ok && k0 + bk < Kh) {
+            const float *row = B + (size_t)(n0 + br) * K;
+            if (INV) {
+                const float4 v0 = *reinterpret_cast<const float4 *>(&row[2 * (k0 + bk)]);
+                const float4 v1 = *reinterpret_cast<const float4 *>(&row[2 * (k0 + bk) + 4]);
+                b0 = make_float4(v0.x, v0.z, v1.x, v1.z);  // b[2k .. 2k+6 even]
+                b1 = make_float4(v0.y, v0.w, v1.y, v1.w);  // odd
+            } else {
+                b0 = *reinterpret_cast<const float4 *>(&row[k0 + bk]);          // b[k .. k+3]
+                b1 = *reinterpret_cast<const float4 *>(&row[K - 4 - k0 - bk]);  // b[K-4-k .. K-1-k]
+            }
         }
     };
     load(0);
-    for (int k0 = 0; k0 < K; k0 += BK16) {
+    for (int k0 = 0; k0 < Kh; k0 += SK16) {
         __syncthreads();
-        *reinterpret_cast<uint4 *>(&sAh[lr][lk]) = ahv;
-        *reinterpret_cast<uint4 *>(&sAl[lr][lk]) = alv;
+        *reinterpret_cast<uint4 *>(&sAh[0][ar][ak]) = aeh;
+        *reinterpret_cast<uint4 *>(&sAl[0][ar][ak]) = ael;
+        *reinterpret_cast<uint4 *>(&sAh[1][ar][ak]) = aoh;
+        *reinterpret_cast<uint4 *>(&sAl[1][ar][ak]) = aol;
         {
-            const float v[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-            uint16_t h[8], l[8];
+            float p[4], q[4];
+            if (INV) {
+                p[0] = b0.x; p[1] = b0.y; p[2] = b0.z; p[3] = b0.w;
+                q[0] = b1.x; q[1] = b1.y; q[2] = b1.z; q[3] = b1.w;
+            } else {
+                const float d[4] = {b0.x - bias, b0.y - bias, b0.z - bias, b0.w - bias};
+                const float mi[4] = {b1.w - bias, b1.z - bias, b1.y - bias, b1.x - bias};  // b[K-1-k-j]
 #pragma unroll
-            for (int e = 0; e < 8; ++e) split16<F16>(F16 ? (v[e] - bias) * bscale : v[e] - bias, h[e], l[e]);
-            uint4 hv, lv;
-            hv.x = h[0] | ((uint32_t)h[1] << 16); hv.y = h[2] | ((uint32_t)h[3] << 16);
-            hv.z = h[4] | ((uint32_t)h[5] << 16); hv.w = h[6] | ((uint32_t)h[7] << 16);
-            lv.x = l[0] | ((uint32_t)l[1] << 16); lv.y = l[2] | ((uint32_t)l[3] << 16);
-            lv.z = l[4] | ((uint32_t)l[5] << 16); lv.w = l[6] | ((uint32_t)l[7] << 16);
-            *reinterpret_cast<uint4 *>(&sBh[lr][lk]) = hv;
-            *reinterpret_cast<uint4 *>(&sBl[lr][lk]) = lv;
+                for (int j = 0; j < 4; ++j) {
+                    p[j] = d[j] + mi[j];
+                    q[j] = d[j] - mi[j];
+                }
+            }
+            if (F16) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    p[j] *= bscale;
+                    q[j] *= bscale;
+                }
+            }
+            split_store4<F16>(p, &sBh[0][br][bk], &sBl[0][br][bk]);
+            split_store4<F16>(q, &sBh[1][br][bk], &sBl[1][br][bk]);
         }
         __syncthreads();
-        if (k0 + BK16 < K) load(k0 + BK16);
+        if (k0 + SK16 < Kh) load(k0 + SK16);
 #pragma unroll
-        for (int ks = 0; ks < BK16; ks += 16) {
+        for (int ks = 0; ks < SK16; ks += 16) {
             const int kk = ks + 8 * (lane >> 5);
-            const uint16_t *xh = &sAh[wm + (lane & 31)][kk], *xl = &sAl[wm + (lane & 31)][kk];
-            const uint16_t *yh = &sBh[wn + (lane & 31)][kk], *yl = &sBl[wn + (lane & 31)][kk];
+            const uint16_t *xh = &sAh[eo][wm + (lane & 31)][kk], *xl = &sAl[eo][wm + (lane & 31)][kk];
+            const uint16_t *yh = &sBh[eo][lane & 31][kk], *yl = &sBl[eo][lane & 31][kk];
             acc = mfma16<F16>(xh, yh, acc);
             acc = mfma16<F16>(xh, yl, acc);
             acc = mfma16<F16>(xl, yh, acc);
         }
     }
-    const float inv = 1.0f / bscale;
-    float m = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = m0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-        const int col = n0 + wn + (lane & 31);
-        if (row < M && col < N) m = fmaxf(m, store_out(a, z, EPI, row, col, acc[i] * inv));
-    }
+    const float m = sym_store<INV, EPI>(a, z, h0, n0, acc, 1.0f / bscale,
+                                        reinterpret_cast<float *>(&sAh[0][0][0]));
     if (F16 && a.cmax) track_max(a.cmax, m);
 }
 
-template <int EPI, bool F16>
-__global__ __launch_bounds__(256) void k_gemm_split(GemmArgs a)
+template <bool INV, int EPI, bool F16>
+__global__ __launch_bounds__(256) void k_sym_split(GemmArgs a)
 {
-    gemm_split_tile<EPI, F16>(a, blockIdx.z, blockIdx.y * BM, blockIdx.x * BN);
+    sym_split_tile<INV, EPI, F16>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
-// Batched products: block b runs tile tiles[b] = (instance, m0, n0, z) with the
+// Batched products: block b runs tile tiles[b] = (instance, h0, n0, z) with the
 // instance's argument block; finished instances are skipped (uniform per block).
-template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_f32_b(const GemmArgs *__restrict__ args,
-                                                    const int4 *__restrict__ tiles)
+template <bool INV, int EPI>
+__global__ __launch_bounds__(256) void k_sym_f32_b(const GemmArgs *__restrict__ args,
+                                                   const int4 *__restrict__ tiles)
 {
     const int4 t = tiles[blockIdx.x];
     const GemmArgs &a = args[t.x];
     if (a.skip && *a.skip) return;
-    gemm_f32_tile<EPI>(a, t.w, t.y, t.z);
+    sym_f32_tile<INV, EPI>(a, t.w, t.y, t.z);
 }
 
-template <int EPI, bool F16>
-__global__ __launch_bounds__(256) void k_gemm_split_b(const GemmArgs *__restrict__ args,
-                                                      const int4 *__restrict__ tiles)
+template <bool INV, int EPI, bool F16>
+__global__ __launch_bounds__(256) void k_sym_split_b(const GemmArgs *__restrict__ args,
+                                                     const int4 *__restrict__ tiles)
 {
     const int4 t = tiles[blockIdx.x];
     const GemmArgs &a = args[t.x];
     if (a.skip && *a.skip) return;
-    gemm_split_tile<EPI, F16>(a, t.w, t.y, t.z);
+    sym_split_tile<INV, EPI, F16>(a, t.w, t.y, t.z);
 }
 
 // ---------------------------------------------------------------- FP64 reference
@@ -402,28 +514,23 @@ __global__ __launch_bounds__(256) void k_field_err(const float *__restrict__ ex,
     }
 }
 
-template <int EPI>
-void launch_split(const GemmArgs &a, dim3 grid, int precision, hipStream_t s)
+// One product (or `batch` independent ones) of the field solve.
+void gemm(const GemmArgs &a, int batch, bool inv, int epi, int precision, hipStream_t s)
 {
-    if (precision == EP_PREC_FP16X3)
-        hipLaunchKernelGGL((k_gemm_split<EPI, true>), grid, dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_gemm_split<EPI, false>), grid, dim3(256), 0, s, a);
-}
-
-void gemm(const GemmArgs &a, int batch, int epi, int precision, hipStream_t s)
-{
-    dim3 grid((a.N + BN - 1) / BN, (a.M + BM - 1) / BM, batch);
-    if (precision == EP_PREC_BF16X3 || precision == EP_PREC_FP16X3) {
-        if (epi == 0)
-            launch_split<0>(a, grid, precision, s);
-        else
-            launch_split<1>(a, grid, precision, s);
+    const dim3 grid((a.N + SBN - 1) / SBN, ((a.M >> 1) + SBM - 1) / SBM, batch);
+    const dim3 blk(256);
+    if (precision == EP_PREC_FP32) {
+        if (inv) hipLaunchKernelGGL((k_sym_f32<true, 0>), grid, blk, 0, s, a);
+        else if (epi) hipLaunchKernelGGL((k_sym_f32<false, 1>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_sym_f32<false, 0>), grid, blk, 0, s, a);
+    } else if (precision == EP_PREC_FP16X3) {
+        if (inv) hipLaunchKernelGGL((k_sym_split<true, 0, true>), grid, blk, 0, s, a);
+        else if (epi) hipLaunchKernelGGL((k_sym_split<false, 1, true>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_sym_split<false, 0, true>), grid, blk, 0, s, a);
     } else {
-        if (epi == 0)
-            hipLaunchKernelGGL(k_gemm_f32<0>, grid, dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL(k_gemm_f32<1>, grid, dim3(256), 0, s, a);
+        if (inv) hipLaunchKernelGGL((k_sym_split<true, 0, false>), grid, blk, 0, s, a);
+        else if (epi) hipLaunchKernelGGL((k_sym_split<false, 1, false>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_sym_split<false, 0, false>), grid, blk, 0, s, a);
     }
 }
 
@@ -470,16 +577,96 @@ float f16_to_f_host(uint16_t u)
     return (float)h;
 }
 
+// Even and odd halves, (M/2) x (M/2) row-major, of one transform matrix:
+//   forward (T[f][x]):  E[r][x] = T[2r][x],  O[r][x] = T[2r+1][x]     (x < M/2)
+//   inverse (T'[x][f]): E[x][r] = T'[x][2r], O[x][r] = T'[x][2r+1]    (x < M/2)
+// with T[f][x] = cos|sin(pi f (x + 1/2) / M) and T' its transpose.
+void halves(int M, bool sine, bool inverse, std::vector<double> &E, std::vector<double> &O)
+{
+    const int h = M / 2;
+    E.assign((size_t)h * h, 0.0);
+    O.assign((size_t)h * h, 0.0);
+    for (int r = 0; r < h; ++r)
+        for (int x = 0; x < h; ++x) {
+            const double ae = M_PI * (2.0 * r) * (x + 0.5) / M, ao = M_PI * (2.0 * r + 1.0) * (x + 0.5) / M;
+            const size_t o = inverse ? (size_t)x * h + r : (size_t)r * h + x;
+            E[o] = sine ? std::sin(ae) : std::cos(ae);
+            O[o] = sine ? std::sin(ao) : std::cos(ao);
+        }
+}
+
 }  // namespace
 
 // Matrix slots (all copies share the index).
 enum { MAT_CY = 0, MAT_CX, MAT_CYT, MAT_SYT, MAT_SXT, MAT_CXT, MAT_N };
 
+// Transform of each slot: sine?, inverse?, and the inverse products' mirror sign.
+constexpr bool kSlotSine[MAT_N] = {false, false, false, true, true, false};
+constexpr bool kSlotInv[MAT_N] = {false, false, true, true, true, true};
+constexpr float kSlotSign[MAT_N] = {1.f, 1.f, 1.f, -1.f, -1.f, 1.f};
+
+// Device copies of one slot's halves ([0] even, [1] odd) in every precision.
+struct HalfMats {
+    float *f32[2] = {};
+    uint16_t *bh[2] = {}, *bl[2] = {};   // BF16 hi/lo
+    uint16_t *fh[2] = {}, *fl[2] = {};   // FP16 hi/lo
+};
+
+static ep_status upload_halves(int M, int slot, HalfMats &hm)
+{
+    std::vector<double> d[2];
+    halves(M, kSlotSine[slot], kSlotInv[slot], d[0], d[1]);
+    for (int k = 0; k < 2; ++k) {
+        const size_t n = d[k].size();
+        std::vector<float> f(n);
+        std::vector<uint16_t> h(n), l(n), h16(n), l16(n);
+        for (size_t e = 0; e < n; ++e) {
+            f[e] = (float)d[k][e];
+            h[e] = bf16_rne_host(f[e]);
+            l[e] = bf16_rne_host(f[e] - bf16_to_f_host(h[e]));
+            h16[e] = f16_host(f[e]);
+            l16[e] = f16_host(f[e] - f16_to_f_host(h16[e]));
+        }
+        if (hipMalloc(&hm.f32[k], n * 4) != hipSuccess || hipMalloc(&hm.bh[k], n * 2) != hipSuccess ||
+            hipMalloc(&hm.bl[k], n * 2) != hipSuccess || hipMalloc(&hm.fh[k], n * 2) != hipSuccess ||
+            hipMalloc(&hm.fl[k], n * 2) != hipSuccess)
+            return fail(EP_ENOMEM, "field: hipMalloc of transform matrices failed");
+        if (hipMemcpy(hm.f32[k], f.data(), n * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(hm.bh[k], h.data(), n * 2, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(hm.bl[k], l.data(), n * 2, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(hm.fh[k], h16.data(), n * 2, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(hm.fl[k], l16.data(), n * 2, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(EP_EDEVICE, "field: upload of transform matrices failed");
+    }
+    return EP_OK;
+}
+
+static void free_halves(HalfMats &hm)
+{
+    for (int k = 0; k < 2; ++k) {
+        void *p[] = {hm.f32[k], hm.bh[k], hm.bl[k], hm.fh[k], hm.fl[k]};
+        for (void *q : p)
+            if (q) (void)hipFree(q);
+    }
+    hm = HalfMats();
+}
+
+// Points product slot z of `a` at matrix slot m in the given precision.
+static void set_operand(GemmArgs &a, int z, const HalfMats &hm, int m, int precision)
+{
+    const bool f16 = precision == EP_PREC_FP16X3;
+    a.A[z] = hm.f32[0];
+    a.Ao[z] = hm.f32[1];
+    a.Ah[z] = f16 ? hm.fh[0] : hm.bh[0];
+    a.Al[z] = f16 ? hm.fl[0] : hm.bl[0];
+    a.Aoh[z] = f16 ? hm.fh[1] : hm.bh[1];
+    a.Aol[z] = f16 ? hm.fl[1] : hm.bl[1];
+    a.sgn[z] = kSlotSign[m];
+}
+
 struct FieldMats {
-    float *f32[MAT_N];
-    uint16_t *hi[MAT_N], *lo[MAT_N];       // BF16 split
-    uint16_t *hi16[MAT_N], *lo16[MAT_N];   // FP16 split
-    double *f64[MAT_N];                    // FP64 reference (built on first use)
+    HalfMats half[MAT_N];                  // even/odd halves, FP32 and split planes
+    double *f64[MAT_N];                    // full FP64 matrices of the reference (built on first use)
     uint32_t *fmax;                        // FP16X3 running maxima: rho, T1t, Eh, Rt
     double *tA64, *tB64, *exd, *eyd;       // FP64 reference scratch
     double *err_part;                      // k_field_err partials
@@ -507,28 +694,8 @@ ep_status field_init(Ctx &c)
         int M;
         bool sine, tr;
         matrix_desc(c, m, M, sine, tr);
-        const std::vector<double> d = trig(M, sine, tr);
-        std::vector<float> f(d.size());
-        std::vector<uint16_t> h(d.size()), l(d.size()), h16(d.size()), l16(d.size());
-        for (size_t k = 0; k < d.size(); ++k) {
-            f[k] = (float)d[k];
-            h[k] = bf16_rne_host(f[k]);
-            l[k] = bf16_rne_host(f[k] - bf16_to_f_host(h[k]));
-            h16[k] = f16_host(f[k]);
-            l16[k] = f16_host(f[k] - f16_to_f_host(h16[k]));
-        }
-        if (hipMalloc(&fm->f32[m], f.size() * 4) != hipSuccess ||
-            hipMalloc(&fm->hi[m], h.size() * 2) != hipSuccess ||
-            hipMalloc(&fm->lo[m], l.size() * 2) != hipSuccess ||
-            hipMalloc(&fm->hi16[m], h.size() * 2) != hipSuccess ||
-            hipMalloc(&fm->lo16[m], l.size() * 2) != hipSuccess)
-            return fail(EP_ENOMEM, "field_init: hipMalloc failed");
-        if (hipMemcpy(fm->f32[m], f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(fm->hi[m], h.data(), h.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(fm->lo[m], l.data(), l.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(fm->hi16[m], h16.data(), h16.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(fm->lo16[m], l16.data(), l16.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
-            return fail(EP_EDEVICE, "field_init: upload failed");
+        ep_status st = upload_halves(M, m, fm->half[m]);
+        if (st != EP_OK) return st;
     }
     // [0,4): split-FP16 operand maxima; [4]: field map mean (float)
     if (hipMalloc(&fm->fmax, 8 * sizeof(uint32_t)) != hipSuccess ||
@@ -544,9 +711,8 @@ void field_free(Ctx &c)
     FieldMats *fm = static_cast<FieldMats *>(c.split_mats);
     if (!fm) return;
     for (int m = 0; m < MAT_N; ++m) {
-        void *p[] = {fm->f32[m], fm->hi[m], fm->lo[m], fm->hi16[m], fm->lo16[m], fm->f64[m]};
-        for (void *q : p)
-            if (q) (void)hipFree(q);
+        free_halves(fm->half[m]);
+        if (fm->f64[m]) (void)hipFree(fm->f64[m]);
     }
     void *p[] = {fm->fmax, fm->tA64, fm->tB64, fm->exd, fm->eyd, fm->err_part};
     for (void *q : p)
@@ -566,36 +732,32 @@ void launch_field(Ctx &c, int precision, hipStream_t s)
     a.scale = (float)(4.0 / ((double)c.Mx * c.My) / ((double)c.bw * c.bh));
     a.wxs = (float)(M_PI / c.Mx);
     a.wys = (float)(M_PI / c.My * ((double)c.bw / c.bh));
-    auto setA = [&](int z, int m) {
-        a.A[z] = fm->f32[m];
-        a.Ah[z] = f16 ? fm->hi16[m] : fm->hi[m];
-        a.Al[z] = f16 ? fm->lo16[m] : fm->lo[m];
-    };
+    auto setA = [&](int z, int m) { set_operand(a, z, fm->half[m], m, precision); };
     if (f16) hipLaunchKernelGGL(k_clear_max, dim3(1), dim3(64), 0, s, fm->fmax + 1, 3);
     // G1: T1t[v][i] = Cy * rho^T
     a.M = c.My; a.N = c.Mx; a.K = c.My;
     setA(0, MAT_CY); a.B[0] = c.map_field; a.C[0] = c.tA;
     a.bmax = fm->fmax + 0; a.cmax = f16 ? fm->fmax + 1 : nullptr;
     a.bbias = c.map_mean;  // the DC mode carries no field; removing it keeps FP32 sums small
-    gemm(a, 1, 0, precision, s);
+    gemm(a, 1, false, 0, precision, s);
     a.bbias = nullptr;
     // G2: A[u][v] = Cx * T1t^T, spectral -> Ehx (tB), Ehy (tB + nb)
     a.M = c.Mx; a.N = c.My; a.K = c.Mx;
     setA(0, MAT_CX); a.B[0] = c.tA; a.C[0] = c.tB; a.C2[0] = c.tB + nb;
     a.bmax = fm->fmax + 1; a.cmax = f16 ? fm->fmax + 2 : nullptr;
-    gemm(a, 1, 1, precision, s);
+    gemm(a, 1, false, 1, precision, s);
     // G3: R1t[j][u] = CyT * Ehx^T, R2t = SyT * Ehy^T
     a.M = c.My; a.N = c.Mx; a.K = c.My;
     setA(0, MAT_CYT); a.B[0] = c.tB; a.C[0] = c.tA;
     setA(1, MAT_SYT); a.B[1] = c.tB + nb; a.C[1] = c.tA + nb;
     a.bmax = fm->fmax + 2; a.cmax = f16 ? fm->fmax + 3 : nullptr;
-    gemm(a, 2, 0, precision, s);
+    gemm(a, 2, true, 0, precision, s);
     // G4: Ex[i][j] = SxT * R1t^T, Ey = CxT * R2t^T
     a.M = c.Mx; a.N = c.My; a.K = c.Mx;
     setA(0, MAT_SXT); a.B[0] = c.tA; a.C[0] = c.ex;
     setA(1, MAT_CXT); a.B[1] = c.tA + nb; a.C[1] = c.ey;
     a.bmax = fm->fmax + 3; a.cmax = nullptr;
-    gemm(a, 2, 0, precision, s);
+    gemm(a, 2, true, 0, precision, s);
 }
 
 // FP64 field of the current map into the reference scratch, then the max-norm
@@ -676,8 +838,7 @@ __global__ void k_clear_max_b(uint32_t *fmax, int n_inst)
 }
 
 struct MatSet {  // the six transform matrices of one grid size
-    float *f32[MAT_N] = {};
-    uint16_t *hi[MAT_N] = {}, *lo[MAT_N] = {}, *hi16[MAT_N] = {}, *lo16[MAT_N] = {};
+    HalfMats half[MAT_N];
 };
 
 }  // namespace
@@ -701,34 +862,11 @@ ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &
     for (const BatchFieldInst &d : inst)
         if (std::find(bf->sizes.begin(), bf->sizes.end(), d.M) == bf->sizes.end()) bf->sizes.push_back(d.M);
     bf->sets.resize(bf->sizes.size());
-    for (size_t si = 0; si < bf->sizes.size(); ++si) {
-        const int M = bf->sizes[si];
+    for (size_t si = 0; si < bf->sizes.size(); ++si)
         for (int m = 0; m < MAT_N; ++m) {
-            const bool sine = (m == MAT_SYT || m == MAT_SXT);
-            const bool tr = !(m == MAT_CY || m == MAT_CX);
-            const std::vector<double> d = trig(M, sine, tr);
-            std::vector<float> f(d.size());
-            std::vector<uint16_t> h(d.size()), l(d.size()), h16(d.size()), l16(d.size());
-            for (size_t k = 0; k < d.size(); ++k) {
-                f[k] = (float)d[k];
-                h[k] = bf16_rne_host(f[k]);
-                l[k] = bf16_rne_host(f[k] - bf16_to_f_host(h[k]));
-                h16[k] = f16_host(f[k]);
-                l16[k] = f16_host(f[k] - f16_to_f_host(h16[k]));
-            }
-            MatSet &s = bf->sets[si];
-            if (hipMalloc(&s.f32[m], f.size() * 4) != hipSuccess || hipMalloc(&s.hi[m], h.size() * 2) != hipSuccess ||
-                hipMalloc(&s.lo[m], l.size() * 2) != hipSuccess || hipMalloc(&s.hi16[m], h.size() * 2) != hipSuccess ||
-                hipMalloc(&s.lo16[m], l.size() * 2) != hipSuccess)
-                return fail(EP_ENOMEM, "batch field: hipMalloc failed");
-            if (hipMemcpy(s.f32[m], f.data(), f.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(s.hi[m], h.data(), h.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(s.lo[m], l.data(), l.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(s.hi16[m], h16.data(), h16.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(s.lo16[m], l16.data(), l16.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
-                return fail(EP_EDEVICE, "batch field: upload failed");
+            ep_status st = upload_halves(bf->sizes[si], m, bf->sets[si].half[m]);
+            if (st != EP_OK) return st;
         }
-    }
     const int n = bf->n_inst;
     if (hipMalloc(&bf->fmax, sizeof(uint32_t) * 4 * std::max(1, n)) != hipSuccess ||
         hipMemset(bf->fmax, 0, sizeof(uint32_t) * 4 * std::max(1, n)) != hipSuccess)
@@ -737,12 +875,13 @@ ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &
     const int precs[3] = {EP_PREC_FP32, EP_PREC_BF16X3, EP_PREC_FP16X3};
     std::vector<int4> tiles[4];
     for (int pi = 0; pi < 3; ++pi) {
-        const bool f16 = precs[pi] == EP_PREC_FP16X3;
+        const int prec = precs[pi];
+        const bool f16 = prec == EP_PREC_FP16X3;
         std::vector<GemmArgs> st[4];
         for (int i = 0; i < n; ++i) {
             const BatchFieldInst &d = inst[i];
             const int si = (int)(std::find(bf->sizes.begin(), bf->sizes.end(), d.M) - bf->sizes.begin());
-            const MatSet &s = bf->sets[si];
+            const MatSet &ms = bf->sets[si];
             const size_t nb = (size_t)d.M * d.M;
             uint32_t *fm = bf->fmax + 4 * i;
             GemmArgs a{};
@@ -751,11 +890,7 @@ ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &
             a.wys = (float)(M_PI / d.M * ((double)d.bw / d.bh));
             a.M = a.N = a.K = d.M;
             a.skip = d.skip;
-            auto setA = [&](GemmArgs &g, int z, int m) {
-                g.A[z] = s.f32[m];
-                g.Ah[z] = f16 ? s.hi16[m] : s.hi[m];
-                g.Al[z] = f16 ? s.lo16[m] : s.lo[m];
-            };
+            auto setA = [&](GemmArgs &g, int z, int m) { set_operand(g, z, ms.half[m], m, prec); };
             GemmArgs g1 = a;
             setA(g1, 0, MAT_CY); g1.B[0] = d.map_field; g1.C[0] = d.tA;
             g1.bmax = d.map_max; g1.cmax = f16 ? fm + 1 : nullptr; g1.bbias = d.map_mean;
@@ -775,11 +910,11 @@ ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &
             st[2].push_back(g3);
             st[3].push_back(g4);
             if (pi == 0) {
-                const int nt = (d.M + BM - 1) / BM;
+                const int mt = ((d.M >> 1) + SBM - 1) / SBM, nt = (d.M + SBN - 1) / SBN;
                 for (int stage = 0; stage < 4; ++stage)
                     for (int z = 0; z < (stage >= 2 ? 2 : 1); ++z)
-                        for (int mt = 0; mt < nt; ++mt)
-                            for (int ntl = 0; ntl < nt; ++ntl) tiles[stage].push_back(make_int4(i, mt * BM, ntl * BN, z));
+                        for (int r = 0; r < mt; ++r)
+                            for (int q = 0; q < nt; ++q) tiles[stage].push_back(make_int4(i, r * SBM, q * SBN, z));
             }
         }
         for (int stage = 0; stage < 4; ++stage) {
@@ -802,12 +937,8 @@ ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &
 void batch_field_free(BatchField *bf)
 {
     if (!bf) return;
-    for (MatSet &s : bf->sets)
-        for (int m = 0; m < MAT_N; ++m) {
-            void *p[] = {s.f32[m], s.hi[m], s.lo[m], s.hi16[m], s.lo16[m]};
-            for (void *q : p)
-                if (q) (void)hipFree(q);
-        }
+    for (MatSet &ms : bf->sets)
+        for (int m = 0; m < MAT_N; ++m) free_halves(ms.half[m]);
     for (int pi = 0; pi < 3; ++pi)
         for (int st = 0; st < 4; ++st)
             if (bf->d_args[pi][st]) (void)hipFree(bf->d_args[pi][st]);
@@ -823,24 +954,22 @@ void launch_field_batch(BatchField *bf, int precision, hipStream_t s)
     if (precision == EP_PREC_FP16X3)
         hipLaunchKernelGGL(k_clear_max_b, dim3((bf->n_inst + 63) / 64), dim3(64), 0, s, bf->fmax, bf->n_inst);
     for (int stage = 0; stage < 4; ++stage) {
-        const dim3 grid(bf->n_tiles[stage]);
+        const dim3 grid(bf->n_tiles[stage]), blk(256);
         const GemmArgs *args = bf->d_args[pi][stage];
         const int4 *tiles = bf->d_tiles[stage];
+        const bool inv = stage >= 2, epi = stage == 1;
         if (precision == EP_PREC_FP32) {
-            if (stage == 1)
-                hipLaunchKernelGGL(k_gemm_f32_b<1>, grid, dim3(256), 0, s, args, tiles);
-            else
-                hipLaunchKernelGGL(k_gemm_f32_b<0>, grid, dim3(256), 0, s, args, tiles);
+            if (inv) hipLaunchKernelGGL((k_sym_f32_b<true, 0>), grid, blk, 0, s, args, tiles);
+            else if (epi) hipLaunchKernelGGL((k_sym_f32_b<false, 1>), grid, blk, 0, s, args, tiles);
+            else hipLaunchKernelGGL((k_sym_f32_b<false, 0>), grid, blk, 0, s, args, tiles);
         } else if (precision == EP_PREC_BF16X3) {
-            if (stage == 1)
-                hipLaunchKernelGGL((k_gemm_split_b<1, false>), grid, dim3(256), 0, s, args, tiles);
-            else
-                hipLaunchKernelGGL((k_gemm_split_b<0, false>), grid, dim3(256), 0, s, args, tiles);
+            if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, false>), grid, blk, 0, s, args, tiles);
+            else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, false>), grid, blk, 0, s, args, tiles);
+            else hipLaunchKernelGGL((k_sym_split_b<false, 0, false>), grid, blk, 0, s, args, tiles);
         } else {
-            if (stage == 1)
-                hipLaunchKernelGGL((k_gemm_split_b<1, true>), grid, dim3(256), 0, s, args, tiles);
-            else
-                hipLaunchKernelGGL((k_gemm_split_b<0, true>), grid, dim3(256), 0, s, args, tiles);
+            if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, true>), grid, blk, 0, s, args, tiles);
+            else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, true>), grid, blk, 0, s, args, tiles);
+            else hipLaunchKernelGGL((k_sym_split_b<false, 0, true>), grid, blk, 0, s, args, tiles);
         }
     }
 }

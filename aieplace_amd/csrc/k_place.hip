@@ -23,12 +23,13 @@ namespace {
 using namespace dev;
 
 
-__global__ void k_fill_maps(float *__restrict__ map_field, float *__restrict__ map_mov,
-                            const float *__restrict__ map_fix, float rho_t, int nb)
+// Clears the fixed-point scatter accumulators (field and movable planes).
+__global__ void k_fill_maps(unsigned long long *__restrict__ acc_field,
+                            unsigned long long *__restrict__ acc_mov, int nb)
 {
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
-        map_field[b] = rho_t * map_fix[b];
-        map_mov[b] = 0.0f;
+        acc_field[b] = 0ull;
+        acc_mov[b] = 0ull;
     }
 }
 
@@ -67,11 +68,13 @@ __global__ __launch_bounds__(kSortScanThreads) void k_sort_scan(int32_t *__restr
     sort_scan_block(count, off, dtab, level, K);
 }
 
+// Writes every cell, packed with its size and movable flag (pack_cell), to its
+// slot of the tile-sorted order.
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(const float2 *__restrict__ pos,
                                                          const float2 *__restrict__ wh, int n_place,
-                                                         GeomArgs g, int Mx, int My,
+                                                         int n_mov, GeomArgs g, int Mx, int My,
                                                          int32_t *__restrict__ off,
-                                                         int32_t *__restrict__ order)
+                                                         float4 *__restrict__ sorted)
 {
     __shared__ int hist[kSortTiles];
     __shared__ int base_of[kSortTiles];
@@ -79,13 +82,16 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const float2 *__restric
     __syncthreads();
     const int base = blockIdx.x * kBlock * kSortCellsPerThread;
     int key[kSortCellsPerThread], rank[kSortCellsPerThread];
+    float4 cell[kSortCellsPerThread];
 #pragma unroll
     for (int k = 0; k < kSortCellsPerThread; ++k) {
         const int idx = base + k * kBlock + threadIdx.x;
         key[k] = -1;
         if (idx < n_place) {
-            key[k] = cell_key(pos[idx], wh[idx], g, Mx, My);
+            const float2 p = pos[idx], s = wh[idx];
+            key[k] = cell_key(p, s, g, Mx, My);
             rank[k] = atomicAdd(&hist[key[k]], 1);
+            cell[k] = pack_cell(p, s, idx < n_mov);
         }
     }
     __syncthreads();
@@ -94,22 +100,19 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const float2 *__restric
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortCellsPerThread; ++k)
-        if (key[k] >= 0) order[base_of[key[k]] + rank[k]] = base + k * kBlock + threadIdx.x;
+        if (key[k] >= 0) sorted[base_of[key[k]] + rank[k]] = cell[k];
 }
 
 // ---- density scatter: block b takes up to kBlock * CPT cells of one super
 // tile (dtab from k_sort_scan) and accumulates them in an LDS window.
 template <int CPT>
-__global__ __launch_bounds__(kBlock) void k_density(const int32_t *__restrict__ order,
+__global__ __launch_bounds__(kBlock) void k_density(const float4 *__restrict__ sorted,
                                                     const int32_t *__restrict__ off,
                                                     const int32_t *__restrict__ dtab, int level,
-                                                    int ext, int wcap, const float2 *__restrict__ pos,
-                                                    const float2 *__restrict__ wh, int n_mov,
-                                                    GeomArgs g, int Mx, int My,
-                                                    float *__restrict__ map_field,
-                                                    float *__restrict__ map_mov)
+                                                    int ext, int wcap, GeomArgs g, int Mx, int My,
+                                                    unsigned long long *__restrict__ acc_field,
+                                                    unsigned long long *__restrict__ acc_mov)
 {
-    extern __shared__ float win[];
     const int ns = kSortTiles >> (2 * level);
     const int b = blockIdx.x;
     if (b >= dtab[ns]) return;
@@ -117,24 +120,33 @@ __global__ __launch_bounds__(kBlock) void k_density(const int32_t *__restrict__ 
     int c0, c1, x0, y0, nx, ny;
     dens_cell_range(off, s, level, b - dtab[s], kBlock * CPT, c0, c1);
     super_window(s, level, Mx, My, ext, x0, y0, nx, ny);
-    density_window<CPT>(order, c0, c1, n_mov, pos, wh, g, Mx, My, x0, y0, nx, ny, map_field,
-                        map_mov, win, wcap);
+    density_window<CPT>(sorted, c0, c1, g, Mx, My, x0, y0, nx, ny, acc_field, acc_mov, wcap);
 }
 
-// Overflow numerator per block: sum_b max(0, mov_b + rho_t*fix_b - rho_t*A_b).
-// Overflow numerator plus the field map's sum, max and min per block (the
-// transforms subtract the map mean; split FP16 scales by the centred range).
+// Converts the fixed-point accumulators into the FP32 maps (field map =
+// rho_t * fixed + smoothed cells; movable map = exact movable overlap) and
+// reduces per block the overflow numerator sum_b max(0, mov_b + rho_t*fix_b -
+// rho_t*A_b) and the field map's sum, max and min (the transforms subtract the
+// map mean; split FP16 scales by the centred range).
 // part layout per block: [ovf, sum, max, -min].
-__global__ __launch_bounds__(kBlock) void k_overflow(const float *__restrict__ map_mov,
-                                                     const float *__restrict__ map_fix,
-                                                     const float *__restrict__ map_field, float rho_t,
-                                                     float cap, int nb, double *__restrict__ part)
+__global__ __launch_bounds__(kBlock) void k_overflow(const unsigned long long *__restrict__ acc_field,
+                                                     const unsigned long long *__restrict__ acc_mov,
+                                                     const float *__restrict__ map_fix, double unit,
+                                                     float rho_t, float cap, int nb,
+                                                     float *__restrict__ map_field,
+                                                     float *__restrict__ map_mov,
+                                                     double *__restrict__ part)
 {
     double acc[4] = {0.0, 0.0, -1e300, -1e300};
     for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock) {
-        const float v = map_mov[b] + rho_t * map_fix[b] - cap;
+        const float fx = map_fix[b];
+        const float mv = (float)((double)acc_mov[b] * unit);
+        const float fd = rho_t * fx + (float)((double)acc_field[b] * unit);
+        map_mov[b] = mv;
+        map_field[b] = fd;
+        const float v = mv + rho_t * fx - cap;
         if (v > 0.0f) acc[0] += (double)v;
-        const double f = map_field[b];
+        const double f = fd;
         acc[1] += f;
         acc[2] = fmax(acc[2], f);
         acc[3] = fmax(acc[3], -f);
@@ -435,7 +447,7 @@ void launch_fill_maps(Ctx &c, hipStream_t s)
 {
     const int nb = c.Mx * c.My;
     hipLaunchKernelGGL(k_fill_maps, dim3(grid_for(nb, kBlock * 4, 4096)), dim3(kBlock), 0, s,
-                       c.map_field, c.map_mov, c.map_fix, c.prm.target_density, nb);
+                       c.acc_field, c.acc_mov, nb);
 }
 
 void launch_sort(Ctx &c, const float2 *pos, hipStream_t s)
@@ -446,23 +458,20 @@ void launch_sort(Ctx &c, const float2 *pos, hipStream_t s)
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortScanThreads), 0, s, c.sort_count, c.sort_off,
                        c.dens_tab, c.dens.level, kBlock * c.dens.cpt);
     hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
-                       geom_args(c), c.Mx, c.My, c.sort_off, c.order);
+                       c.n_mov, geom_args(c), c.Mx, c.My, c.sort_off, c.sorted);
 }
 
-void launch_density(Ctx &c, const float2 *pos, hipStream_t s)
+void launch_density(Ctx &c, hipStream_t s)
 {
     const DensPlan &dp = c.dens;
     const int blocks = dens_grid(dp, c.n_place);
-    const size_t lds = 2 * sizeof(float) * dp.wcap;
+    const size_t lds = dens_lds_bytes(dp);
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, s, c.order, c.sort_off, c.dens_tab,
-                           dp.level, dp.ext, dp.wcap, pos, c.wh, c.n_mov, geom_args(c), c.Mx, c.My,
-                           c.map_field, c.map_mov);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, s, c.sorted, c.sort_off, c.dens_tab,
+                           dp.level, dp.ext, dp.wcap, geom_args(c), c.Mx, c.My, c.acc_field, c.acc_mov);
     };
     if (dp.cpt == 2) go(k_density<2>);
     else if (dp.cpt == 8) go(k_density<8>);
-    else if (dp.cpt == 16) go(k_density<16>);
-    else if (dp.cpt == 32) go(k_density<32>);
     else go(k_density<4>);
 }
 
@@ -470,8 +479,10 @@ void launch_overflow(Ctx &c, hipStream_t s)
 {
     const int nb = c.Mx * c.My;
     const float cap = c.prm.target_density * c.bw * c.bh;
-    hipLaunchKernelGGL(k_overflow, dim3(c.ovf_blocks), dim3(kBlock), 0, s, c.map_mov, c.map_fix,
-                       c.map_field, c.prm.target_density, cap, nb, c.part + c.part_ovf_off);
+    const double unit = (double)c.bw * (double)c.bh / kDensFixUnits;
+    hipLaunchKernelGGL(k_overflow, dim3(c.ovf_blocks), dim3(kBlock), 0, s, c.acc_field, c.acc_mov,
+                       c.map_fix, unit, c.prm.target_density, cap, nb, c.map_field, c.map_mov,
+                       c.part + c.part_ovf_off);
     hipLaunchKernelGGL(k_map_stats, dim3(1), dim3(kBlock), 0, s, c.part + c.part_ovf_off,
                        c.ovf_blocks, nb, c.map_mean, c.field_max);
 }

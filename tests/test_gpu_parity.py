@@ -244,3 +244,24 @@ def test_graph_replay_matches_eager(oracle):
     for a, b in zip(recs[0], recs[1]):
         assert a["hpwl"] == pytest.approx(b["hpwl"], rel=1e-3)
         assert a["overflow"] == pytest.approx(b["overflow"], rel=1e-3, abs=1e-4)
+
+
+@pytest.mark.parametrize("precision", [api.EP_PREC_FP32, api.EP_PREC_BF16X3])
+def test_placement_is_deterministic(precision):
+    """The density maps accumulate in 64-bit fixed point and every other
+    reduction runs in a fixed order, so two runs of the same placement give
+    bit-identical trajectories, positions and HPWL."""
+    say(f"case determinism precision={precision}")
+    nl, p = make("stdcell", 20_000, 128, use_graph=1, precision=precision)
+    out = []
+    for _ in range(2):
+        with A.Placer(nl, p) as pl:
+            r = pl.run(log_cap=2000)
+            x, y = pl.get_positions()
+        out.append((r, x, y))
+    (r1, x1, y1), (r2, x2, y2) = out
+    say(f"  it {r1['iterations']} / {r2['iterations']}  hpwl {r1['hpwl']!r} / {r2['hpwl']!r}")
+    assert r1["iterations"] == r2["iterations"] and r1["evals"] == r2["evals"]
+    assert r1["hpwl"] == r2["hpwl"] and r1["overflow"] == r2["overflow"]
+    assert [l["hpwl"] for l in r1["log"]] == [l["hpwl"] for l in r2["log"]]
+    assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
