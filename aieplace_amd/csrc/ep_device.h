@@ -14,6 +14,31 @@ namespace ep {
 namespace dev {
 
 constexpr int kSortScanThreads = 1024;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
+
+// Workgroups are dispatched to the XCDs round-robin (physical block b runs on
+// XCD b % 8).  Returns the logical block of physical block b in a grid of n so
+// that each XCD runs one contiguous range of logical blocks: neighbouring
+// blocks, which gather from the same nets, cells and bins, then share an L2.
+__device__ inline int xcd_block(int b, int n)
+{
+    const int q = n / kXcds, r = n % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
+// As xcd_block, for grids whose work varies along the index range: the grid is
+// cut into groups of kXcds * S consecutive logical blocks and each XCD takes
+// one run of S blocks from every group (runs stay within one L2, and every XCD
+// sees the whole range, so no XCD gets only the heavy or idle blocks).  Blocks
+// past the last whole group keep their index.
+__device__ inline int xcd_run_block(int b, int n, int S)
+{
+    const int G = kXcds * S, full = n / G * G;
+    if (b >= full) return b;
+    const int x = b % kXcds, k = b / kXcds;
+    return ((k / S) * kXcds + x) * S + (k % S);
+}
+constexpr int kXcdRun = 16;
 
 struct Geom {
     float xl, yl, xh, yh, bw, bh, sqrt2_bw, sqrt2_bh;

@@ -188,14 +188,15 @@ __global__ __launch_bounds__(kBlock) void kb_sort_scatter(BDev d, int at)
 template <int CPT>
 __global__ __launch_bounds__(kBlock) void kb_density(BDev d)
 {
-    const int i = find_inst(d.pre_dens, d.n_inst, blockIdx.x);
+    const int lb = xcd_run_block(blockIdx.x, gridDim.x, kXcdRun);
+    const int i = find_inst(d.pre_dens, d.n_inst, lb);
     const BState *S = d.st + i;
     if (S->done) return;
     const BInst &I = d.inst[i];
     const int level = I.dens_level;
     const int32_t *dtab = d.dens_tab + (size_t)i * (kSortTiles + 1);
     const int ns = kSortTiles >> (2 * level);
-    const int b = blockIdx.x - d.pre_dens[i];
+    const int b = lb - d.pre_dens[i];
     if (b >= dtab[ns]) return;
     const int s = dens_super_tile(dtab, ns, b);
     int c0, c1, x0, y0, nx, ny;
@@ -277,10 +278,11 @@ __device__ inline void wa_tile(const BDev &d, const BBucket &bk, int blk, const 
 
 __global__ __launch_bounds__(kBlock) void kb_wa(BDev d, int at)
 {
+    const int lb = xcd_run_block(blockIdx.x, gridDim.x, kXcdRun);
     int lo = 0, hi = d.n_buckets;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (d.buckets[mid].blk0 <= (int)blockIdx.x) lo = mid;
+        if (d.buckets[mid].blk0 <= lb) lo = mid;
         else hi = mid;
     }
     const BBucket bk = d.buckets[lo];
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void kb_wa(BDev d, int at)
     if (S->done) return;
     const float2 *pos = eval_pos(d, S->cur, at);
     const float ginv = S->f_ginv;
-    const int blk = blockIdx.x - bk.blk0;
+    const int blk = lb - bk.blk0;
     double hp = 0.0, wa = 0.0;
     switch (bk.lanes) {
     case 2: wa_tile<2>(d, bk, blk, pos, ginv, hp, wa); break;
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void kb_wa(BDev d, int at)
     default: wa_tile<64>(d, bk, blk, pos, ginv, hp, wa); break;
     }
     double v[2] = {hp, wa};
-    block_reduce<2, 0>(v, d.part_wa + 2 * (size_t)blockIdx.x);
+    block_reduce<2, 0>(v, d.part_wa + 2 * (size_t)lb);
 }
 
 // ------------------------------------------------------------------ per-cell gather
@@ -309,7 +311,8 @@ __global__ __launch_bounds__(kBlock) void kb_wa(BDev d, int at)
 // at == kAtCur:   position V[cur], no references, output G[cur].
 __global__ __launch_bounds__(kBlock) void kb_cell_grad(BDev d, int at)
 {
-    const int i = find_inst(d.pre_cell, d.n_inst, blockIdx.x);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int i = find_inst(d.pre_cell, d.n_inst, lb);
     const BState *S = d.st + i;
     if (S->done) return;
     const BInst &I = d.inst[i];
@@ -321,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void kb_cell_grad(BDev d, int at)
     const int M = I.M;
     const float *ex = d.ex + I.map_off, *ey = d.ey + I.map_off;
     const float area_scale = g.v[4] * g.v[5];
-    const int lc = (blockIdx.x - d.pre_cell[i]) * kBlock + threadIdx.x;
+    const int lc = (lb - d.pre_cell[i]) * kBlock + threadIdx.x;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (lc < I.n_place) {
         const int c = I.cell_off + lc;
@@ -369,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void kb_cell_grad(BDev d, int at)
         if (!isfinite(gx) || !isfinite(gy)) acc[5] += 1.0;
         acc[6] = fmax(fabs((double)gx), fabs((double)gy));
     }
-    block_reduce<7, 1>(acc, d.part_cell + 7 * (size_t)blockIdx.x);
+    block_reduce<7, 1>(acc, d.part_cell + 7 * (size_t)lb);
 }
 
 // ------------------------------------------------------------------ reduction + control

@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4 *__restrict__ s
                                                     unsigned long long *__restrict__ acc_mov)
 {
     const int ns = kSortTiles >> (2 * level);
-    const int b = blockIdx.x;
+    const int b = xcd_run_block(blockIdx.x, gridDim.x, kXcdRun);  // neighbouring super tiles share an L2
     if (b >= dtab[ns]) return;
     const int s = dens_super_tile(dtab, ns, b);
     int c0, c1, x0, y0, nx, ny;
@@ -207,8 +207,10 @@ __global__ __launch_bounds__(kBlock) void k_wa(BucketTable bt, const int32_t *__
 {
     const float ginv = sc->gamma_inv;
     double hp = 0.0, wa = 0.0;
-    // virtual blocks: the launch is capped, each block walks several tiles of nets
-    for (int vb = blockIdx.x; vb < total_blocks; vb += gridDim.x) {
+    // one net tile per block (virtual blocks if the launch is capped); runs of
+    // neighbouring tiles share an XCD's L2
+    for (int vb = xcd_run_block(blockIdx.x, gridDim.x, kXcdRun); vb < total_blocks;
+         vb += gridDim.x) {
         int b = 0;
         while (b + 1 < bt.n && vb >= bt.b[b + 1].blk0) ++b;
         const Bucket bk = bt.b[b];
@@ -280,7 +282,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
     const float area_scale = g.v[4] * g.v[5];
     // sums: dv2, dg2, |gw|, |f|, g2, nonfinite ; max: |g|
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int c = blockIdx.x * kBlock + threadIdx.x; c < n_place; c += gridDim.x * kBlock) {
+    for (int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; c < n_place;
+         c += gridDim.x * kBlock) {
         const float2 p = pos[c], s = wh[c];
         float gwx = 0.f, gwy = 0.f;
         int np = 0;
