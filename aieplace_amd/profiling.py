@@ -33,7 +33,7 @@ PREC_NAME = {0: "fp32", 1: "bf16x3", 2: "fp16x3"}
 # kernel families of one gradient evaluation (name prefix -> ep_profile item)
 FAMILIES = [
     ("k_sort_", "cell_sort"), ("k_density", "density_scatter"), ("k_overflow", "overflow"),
-    ("k_map_stats", "overflow"), ("k_wa", "wa_grad"), ("k_gemm_", "field_transforms"),
+    ("k_map_stats", "overflow"), ("k_wa", "wa_grad"), ("k_sym_", "field_transforms"),
     ("k_clear_max", "field_transforms"), ("k_cell_grad", "cell_grad"), ("k_trial", "nesterov_trial"),
     ("k_finalize", "finalize"), ("k_fill_maps", "fill_maps"),
 ]
