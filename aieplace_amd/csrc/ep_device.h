@@ -511,6 +511,79 @@ __device__ inline void density_window(const float4 *__restrict__ sorted, int c0,
     }
 }
 
+// ------------------------------------------------------------------ per-cell gradient
+
+// Sum of a cell's pin partials pin_grad[cpins[k0 .. k1)], in order, with up to
+// four gathers in flight per step.
+__device__ inline float2 sum_pin_partials(int k0, int k1, const int32_t *__restrict__ cpins,
+                                          const float2 *__restrict__ pin_grad)
+{
+    float gx = 0.f, gy = 0.f;
+    for (int k = k0; k < k1; k += 4) {
+        int q[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = k + j < k1 ? cpins[k + j] : -1;
+        float2 v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = q[j] >= 0 ? pin_grad[q[j]] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            gx += v[j].x;
+            gy += v[j].y;
+        }
+    }
+    return make_float2(gx, gy);
+}
+
+// Density force on a cell: r * sum over its smoothed rectangle of overlap * E
+// (ePlace local smoothing, the adjoint of the field-map scatter).  Rows first,
+// the y overlaps shared by every row, so the field gathers of a row issue
+// together.
+__device__ inline float2 cell_force(float2 p, float2 s, const float *__restrict__ ex,
+                                    const float *__restrict__ ey, const GeomArgs &g, int Mx, int My)
+{
+    Rect r;
+    stretched_rect(p, s, g, Mx, My, r);
+    float fx = 0.f, fy = 0.f;
+    if (r.i0 <= r.i1 && r.j0 <= r.j1) {
+        const int dj = r.j1 - r.j0;
+        if (dj < 4) {
+            float oy[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) oy[q] = q <= dj ? ovl(r.ty0, r.ty1, r.j0 + q) : 0.0f;
+            for (int i = r.i0; i <= r.i1; ++i) {
+                const float ox = ovl(r.tx0, r.tx1, i);
+                const size_t base = (size_t)i * My + r.j0;
+                float vx[4], vy[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    vx[q] = q <= dj ? ex[base + q] : 0.0f;
+                    vy[q] = q <= dj ? ey[base + q] : 0.0f;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q <= dj) {
+                        const float o = ox * oy[q];
+                        fx += o * vx[q];
+                        fy += o * vy[q];
+                    }
+            }
+        } else {
+            for (int i = r.i0; i <= r.i1; ++i) {
+                const float ox = ovl(r.tx0, r.tx1, i);
+                for (int j = r.j0; j <= r.j1; ++j) {
+                    const size_t b = (size_t)i * My + j;
+                    const float o = ox * ovl(r.ty0, r.ty1, j);
+                    fx += o * ex[b];
+                    fy += o * ey[b];
+                }
+            }
+        }
+    }
+    const float fs = r.ratio * g.v[4] * g.v[5];
+    return make_float2(fx * fs, fy * fs);
+}
+
 __device__ inline float2 clamp_cell(float2 p, float2 s, const GeomArgs &g)
 {
     p.x = fminf(fmaxf(p.x, g.v[0]), g.v[2] - s.x);

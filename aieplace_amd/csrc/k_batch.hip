@@ -323,51 +323,30 @@ __global__ __launch_bounds__(kBlock) void kb_cell_grad(BDev d, int at)
     const GeomArgs g = inst_geom(I);
     const int M = I.M;
     const float *ex = d.ex + I.map_off, *ey = d.ey + I.map_off;
-    const float area_scale = g.v[4] * g.v[5];
     const int lc = (lb - d.pre_cell[i]) * kBlock + threadIdx.x;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (lc < I.n_place) {
         const int c = I.cell_off + lc;
         const float2 p = pos[c], s = d.wh[c];
-        float gwx = 0.f, gwy = 0.f;
+        float2 v0 = p, g0 = make_float2(0.f, 0.f);  // independent loads first
+        if (at == kAtTrial) {
+            v0 = pick<const float2 *>(d.V[0], d.V[1], a)[c];
+            g0 = pick<const float2 *>(d.G[0], d.G[1], a)[c];
+        }
         const int k0 = d.cps[c], k1 = d.cps[c + 1];
-        for (int k = k0; k < k1; ++k) {
-            const float2 pg = d.pin_grad[d.cpins[k]];
-            gwx += pg.x;
-            gwy += pg.y;
-        }
-        const int np = k1 - k0;
-        Rect r;
-        stretched_rect(p, s, g, M, M, r);
-        float fx = 0.f, fy = 0.f;
-        for (int bi = r.i0; bi <= r.i1; ++bi) {
-            const float ox = ovl(r.tx0, r.tx1, bi);
-            if (ox <= 0.0f) continue;
-            for (int bj = r.j0; bj <= r.j1; ++bj) {
-                const float oy = ovl(r.ty0, r.ty1, bj);
-                if (oy <= 0.0f) continue;
-                const size_t o = (size_t)bi * M + bj;
-                const float w = ox * oy;
-                fx += w * ex[o];
-                fy += w * ey[o];
-            }
-        }
-        const float fs = r.ratio * area_scale;
-        fx *= fs;
-        fy *= fs;
-        const float pre = fmaxf(1.0f, (float)np + lambda * s.x * s.y);
-        const float gx = (gwx - lambda * fx) / pre, gy = (gwy - lambda * fy) / pre;
+        const float2 f = cell_force(p, s, ex, ey, g, M, M);
+        const float2 gw = sum_pin_partials(k0, k1, d.cpins, d.pin_grad);
+        const float pre = fmaxf(1.0f, (float)(k1 - k0) + lambda * s.x * s.y);
+        const float gx = (gw.x - lambda * f.x) / pre, gy = (gw.y - lambda * f.y) / pre;
         gout[c] = make_float2(gx, gy);
         if (at == kAtTrial) {
-            const float2 v0 = pick<const float2 *>(d.V[0], d.V[1], a)[c];
             const double dx = (double)p.x - v0.x, dy = (double)p.y - v0.y;
             acc[0] += dx * dx + dy * dy;
-            const float2 g0 = pick<const float2 *>(d.G[0], d.G[1], a)[c];
             const double ddx = (double)gx - g0.x, ddy = (double)gy - g0.y;
             acc[1] += ddx * ddx + ddy * ddy;
         }
-        acc[2] += fabs((double)gwx) + fabs((double)gwy);
-        acc[3] += fabs((double)fx) + fabs((double)fy);
+        acc[2] += fabs((double)gw.x) + fabs((double)gw.y);
+        acc[3] += fabs((double)f.x) + fabs((double)f.y);
         acc[4] += (double)gx * gx + (double)gy * gy;
         if (!isfinite(gx) || !isfinite(gy)) acc[5] += 1.0;
         acc[6] = fmax(fabs((double)gx), fabs((double)gy));

@@ -279,53 +279,32 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
     float2 *__restrict__ f_out, const DevScalars *__restrict__ sc, double *__restrict__ part)
 {
     const float lambda = sc->lambda;
-    const float area_scale = g.v[4] * g.v[5];
     // sums: dv2, dg2, |gw|, |f|, g2, nonfinite ; max: |g|
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; c < n_place;
          c += gridDim.x * kBlock) {
         const float2 p = pos[c], s = wh[c];
-        float gwx = 0.f, gwy = 0.f;
-        int np = 0;
+        const float2 v0 = vref ? vref[c] : p;  // independent loads first
+        const float2 g0 = gref ? gref[c] : make_float2(0.f, 0.f);
+        int k0 = 0, k1 = 0;
         if (c < n_mov) {
-            const int k0 = cell_pin_start[c], k1 = cell_pin_start[c + 1];
-            np = k1 - k0;
-            for (int k = k0; k < k1; ++k) {
-                const float2 pg = pin_grad[cell_pins[k]];
-                gwx += pg.x;
-                gwy += pg.y;
-            }
+            k0 = cell_pin_start[c];
+            k1 = cell_pin_start[c + 1];
         }
-        Rect r;
-        stretched_rect(p, s, g, Mx, My, r);
-        float fx = 0.f, fy = 0.f;
-        for (int i = r.i0; i <= r.i1; ++i) {
-            const float ox = ovl(r.tx0, r.tx1, i);
-            if (ox <= 0.0f) continue;
-            for (int j = r.j0; j <= r.j1; ++j) {
-                const float oy = ovl(r.ty0, r.ty1, j);
-                if (oy <= 0.0f) continue;
-                const size_t b = (size_t)i * My + j;
-                const float o = ox * oy;
-                fx += o * ex[b];
-                fy += o * ey[b];
-            }
-        }
-        const float fs = r.ratio * area_scale;
-        fx *= fs;
-        fy *= fs;
+        const int np = k1 - k0;
+        const float2 f = cell_force(p, s, ex, ey, g, Mx, My);
+        const float2 gw = sum_pin_partials(k0, k1, cell_pins, pin_grad);
+        const float gwx = gw.x, gwy = gw.y, fx = f.x, fy = f.y;
         const float pre = fmaxf(1.0f, (float)np + lambda * s.x * s.y);
         const float gx = (gwx - lambda * fx) / pre, gy = (gwy - lambda * fy) / pre;
         gout[c] = make_float2(gx, gy);
         if (gw_out) gw_out[c] = make_float2(gwx, gwy);
         if (f_out) f_out[c] = make_float2(fx, fy);
         if (vref) {
-            const float2 v0 = vref[c];
             const double dx = (double)p.x - v0.x, dy = (double)p.y - v0.y;
             acc[0] += dx * dx + dy * dy;
         }
         if (gref) {
-            const float2 g0 = gref[c];
             const double dx = (double)gx - g0.x, dy = (double)gy - g0.y;
             acc[1] += dx * dx + dy * dy;
         }
