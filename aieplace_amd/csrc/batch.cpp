@@ -146,10 +146,12 @@ struct ep_batch {
         return d;
     }
 
-    // One gradient evaluation of every unfinished instance.
-    void eval(const BDev &d, int at, hipStream_t s)
+    // One gradient evaluation of every unfinished instance (`counted`: the
+    // preceding trial move counted the sort tiles).
+    void eval(const BDev &d, int at, bool counted, hipStream_t s)
     {
-        bl_maps(d, at, bin_blocks, sort_blocks, dens_blocks[dens_variant], kDensCpt[dens_variant], s);
+        bl_maps(d, at, counted, bin_blocks, sort_blocks, dens_blocks[dens_variant],
+                kDensCpt[dens_variant], s);
         bl_wa(d, at, wa_blocks, s);
         launch_field_batch(bf, prm.precision, s);
         bl_cell_grad(d, at, cell_blocks, s);
@@ -220,10 +222,11 @@ static ep_status autotune(ep_batch *b)
     for (int v = 0; v < 3; ++v) {
         b->dens_variant = v;
         const BDev d = b->dev();
-        bl_maps(d, kAtCur, b->bin_blocks, b->sort_blocks, b->dens_blocks[v], kDensCpt[v], b->stream);
+        bl_maps(d, kAtCur, false, b->bin_blocks, b->sort_blocks, b->dens_blocks[v], kDensCpt[v], b->stream);
         (void)hipEventRecord(e0, b->stream);
         for (int r = 0; r < kTuneReps; ++r)
-            bl_maps(d, kAtCur, b->bin_blocks, b->sort_blocks, b->dens_blocks[v], kDensCpt[v], b->stream);
+            bl_maps(d, kAtCur, false, b->bin_blocks, b->sort_blocks, b->dens_blocks[v], kDensCpt[v],
+                    b->stream);
         (void)hipEventRecord(e1, b->stream);
         hipError_t e = hipEventSynchronize(e1);
         if (e != hipSuccess) {
@@ -322,7 +325,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
         for (int k = 0; k < L.buckets.n; ++k) {
             const Bucket &bk = L.buckets.b[k];
             bks.push_back({i, bk.lanes, (int32_t)(net_base + bk.net0), (int32_t)(net_base + bk.net1),
-                           (int32_t)(wa_base + bk.blk0)});
+                           (int32_t)(wa_base + bk.blk0), bk.deg, (int32_t)(pin_base + bk.pin0)});
         }
         // cell -> pin CSR over all cells (empty for fillers, fixed and padding)
         for (int64_t k = 0; k < span; ++k)
@@ -496,8 +499,8 @@ extern "C" ep_status ep_batch_init(ep_batch *b)
     if (b->ready) return EP_OK;
     const BDev d = b->dev();
     hipStream_t s = b->stream;
-    bl_move(d, kMoveInitU, b->cell_blocks, s);
-    bl_maps(d, kAtCur, b->bin_blocks, b->sort_blocks, b->dens_blocks[b->dens_variant],
+    bl_move(d, kMoveInitU, b->sort_blocks, s);
+    bl_maps(d, kAtCur, false, b->bin_blocks, b->sort_blocks, b->dens_blocks[b->dens_variant],
             kDensCpt[b->dens_variant], s);
     bl_reduce(d, s);
     bl_control(d, kCtlGamma, s);
@@ -511,8 +514,8 @@ extern "C" ep_status ep_batch_init(ep_batch *b)
     bl_cell_grad(d, kAtCur, b->cell_blocks, s);
     bl_reduce(d, s);
     bl_control(d, kCtlProbe, s);
-    bl_move(d, kMoveProbe, b->cell_blocks, s);
-    b->eval(d, kAtTrial, s);
+    bl_move(d, kMoveProbe, b->sort_blocks, s);
+    b->eval(d, kAtTrial, false, s);
     bl_control(d, kCtlAlpha, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(EP_EDEVICE, "ep_batch_init: launch failed: %s", hipGetErrorString(e));
@@ -529,8 +532,8 @@ static ep_status build_graph(ep_batch *b)
     hipGraph_t g = nullptr;
     EPB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int t = 0; t < kTrialsPerLaunch; ++t) {
-        bl_move(d, kMoveTrial, b->cell_blocks, b->stream);
-        b->eval(d, kAtTrial, b->stream);
+        bl_move(d, kMoveTrial, b->sort_blocks, b->stream);
+        b->eval(d, kAtTrial, true, b->stream);
         bl_control(d, kCtlIterate, b->stream);
     }
     (void)hipMemcpyAsync(b->h_st, b->d_st, sizeof(BState) * b->n, hipMemcpyDeviceToHost, b->stream);

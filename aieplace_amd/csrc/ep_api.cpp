@@ -87,11 +87,12 @@ void push_scalars(Ctx &c, hipStream_t s)
 }
 
 // One gradient evaluation at `pos`: density + overflow, WA, field, gather.
+// `counted`: a counting launch_trial produced pos (its sort counts are made).
 void enqueue_eval(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref, float2 *gout,
-                  float2 *gw_out, float2 *f_out, hipStream_t s)
+                  float2 *gw_out, float2 *f_out, hipStream_t s, bool counted = false)
 {
     launch_fill_maps(c, s);
-    launch_sort(c, pos, s);
+    launch_sort(c, pos, counted, s);
     launch_density(c, s);
     launch_overflow(c, s);
     launch_wa(c, pos, s);
@@ -117,8 +118,8 @@ ep_status trial_eval(Ctx &c)
             EP_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
             push_scalars(c, c.stream);
             launch_mlp_propose(c, c.stream);
-            launch_trial(c, c.V[a], c.U[a], c.G[a], c.V[b], c.U[b], c.stream);
-            enqueue_eval(c, c.V[b], c.V[a], c.G[a], c.G[b], nullptr, nullptr, c.stream);
+            launch_trial(c, c.V[a], c.U[a], c.G[a], c.V[b], c.U[b], true, c.stream);
+            enqueue_eval(c, c.V[b], c.V[a], c.G[a], c.G[b], nullptr, nullptr, c.stream, true);
             EP_HIP(hipStreamEndCapture(c.stream, &g));
             hipError_t e = hipGraphInstantiate(&c.graph_exec[a], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -129,8 +130,8 @@ ep_status trial_eval(Ctx &c)
     } else {
         push_scalars(c, c.stream);
         launch_mlp_propose(c, c.stream);
-        launch_trial(c, c.V[a], c.U[a], c.G[a], c.V[b], c.U[b], c.stream);
-        enqueue_eval(c, c.V[b], c.V[a], c.G[a], c.G[b], nullptr, nullptr, c.stream);
+        launch_trial(c, c.V[a], c.U[a], c.G[a], c.V[b], c.U[b], true, c.stream);
+        enqueue_eval(c, c.V[b], c.V[a], c.G[a], c.G[b], nullptr, nullptr, c.stream, true);
         ep_status st = check_launch();
         if (st != EP_OK) return st;
     }
@@ -380,7 +381,10 @@ ep_status ep::build_layout(const ep_netlist *nl, const ep_params *p, HostLayout 
     const int N = nl->n_nets;
     L.n_nets = N;
     L.n_pins = nl->n_pins;
-    const int lane_classes[kMaxBuckets] = {2, 4, 8, 16, 32, 64};
+    // classes: exact degrees 2, 3, 4 (their pin ranges follow from the net's
+    // rank in the class), then lane widths 8, 16, 32, 64
+    const int lane_classes[kMaxBuckets] = {2, 4, 4, 8, 16, 32, 64};
+    const int class_deg[kMaxBuckets] = {2, 3, 4, 0, 0, 0, 0};
     std::vector<std::vector<int32_t>> cls(kMaxBuckets + 1);
     for (int n = 0; n < N; ++n) {
         int d = nl->net_start[n + 1] - nl->net_start[n];
@@ -388,8 +392,11 @@ ep_status ep::build_layout(const ep_netlist *nl, const ep_params *p, HostLayout 
             cls[kMaxBuckets].push_back(n);
             continue;
         }
-        int g = lanes_for_degree(d), b = 0;
-        while (lane_classes[b] != g) ++b;
+        int b = d <= 4 ? d - 2 : 3;
+        if (d > 4) {
+            const int g = lanes_for_degree(d);
+            while (lane_classes[b] != g) ++b;
+        }
         cls[b].push_back(n);
     }
     // within a class, order nets by their lowest cell index: nets on nearby
@@ -416,6 +423,8 @@ ep_status ep::build_layout(const ep_netlist *nl, const ep_params *p, HostLayout 
         if (b < kMaxBuckets && !cls[b].empty()) {
             Bucket &bk = L.buckets.b[L.buckets.n++];
             bk.lanes = lane_classes[b];
+            bk.deg = class_deg[b];
+            bk.pin0 = pin_w;
             bk.net0 = net_w;
             bk.net1 = net_w + (int32_t)cls[b].size();
             bk.blk0 = blk;
@@ -670,7 +679,7 @@ extern "C" ep_status ep_get_positions(ep_ctx *c, float *x, float *y)
 static ep_status run_density(ep_ctx *c)
 {
     launch_fill_maps(*c, c->stream);
-    launch_sort(*c, c->V[c->cur], c->stream);
+    launch_sort(*c, c->V[c->cur], false, c->stream);
     launch_density(*c, c->stream);
     launch_overflow(*c, c->stream);
     launch_finalize(*c, c->stream);
@@ -1075,7 +1084,7 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
     push_scalars(*c, c->stream);
     // field input must be a real map: build it once at the current positions
     launch_fill_maps(*c, c->stream);
-    launch_sort(*c, c->V[a], c->stream);
+    launch_sort(*c, c->V[a], false, c->stream);
     launch_density(*c, c->stream);
     launch_overflow(*c, c->stream);
     hipEvent_t e0, e1;
@@ -1103,6 +1112,7 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
         {"wa_grad", 1, 4.0 * (N + 1) + 12.0 * P + 8.0 * c->n_all + 8.0 * P, 0},
         {"field_transforms", 4, field_bytes, gemm_flops},
         {"cell_grad", 1, 36.0 * np + 12.0 * pm + 4.0 * c->n_mov + 8.0 * nb, 0},
+        // v, u, g, size read; v, u written (the tile counts stay in LDS)
         {"nesterov_trial", 1, 40.0 * np, 0},
         {"finalize", 1, 8.0 * c->part_total, 0},
         {"fill_maps", 1, 16.0 * nb, 0},
@@ -1112,17 +1122,20 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
         EP_HIP(hipEventRecord(e0, c->stream));
         for (int r = 0; r < reps; ++r) {
             const std::string nm = it.name;
-            if (nm == "cell_sort") launch_sort(*c, c->V[a], c->stream);
+            if (nm == "cell_sort") launch_sort(*c, c->V[a], false, c->stream);
             else if (nm == "density_scatter") launch_density(*c, c->stream);
             else if (nm == "overflow") launch_overflow(*c, c->stream);
             else if (nm == "wa_grad") launch_wa(*c, c->V[a], c->stream);
             else if (nm == "field_transforms") launch_field(*c, c->prm.precision, c->stream);
             else if (nm == "cell_grad") launch_cell_grad(*c, c->V[a], c->V[a], c->G[a], c->G[b], nullptr, nullptr, c->stream);
-            else if (nm == "nesterov_trial") launch_trial(*c, c->V[a], c->U[a], c->G[a], c->V[b], c->U[b], c->stream);
+            else if (nm == "nesterov_trial") launch_trial(*c, c->V[a], c->U[a], c->G[a], c->V[b], c->U[b], true, c->stream);
             else if (nm == "finalize") launch_finalize(*c, c->stream);
             else launch_fill_maps(*c, c->stream);
         }
         EP_HIP(hipEventRecord(e1, c->stream));
+        // the counting trials left tile counts behind: consume them (the scan
+        // clears the counts) before anything else sorts
+        if (std::string(it.name) == "nesterov_trial") launch_sort(*c, c->V[b], true, c->stream);
         EP_HIP(hipEventSynchronize(e1));
         ep_status st = check_launch();
         if (st != EP_OK) return st;
@@ -1143,7 +1156,7 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
     *n = k;
     // leave the maps consistent with the current positions
     launch_fill_maps(*c, c->stream);
-    launch_sort(*c, c->V[a], c->stream);
+    launch_sort(*c, c->V[a], false, c->stream);
     launch_density(*c, c->stream);
     launch_overflow(*c, c->stream);
     return sync(*c);

@@ -67,6 +67,7 @@ struct BCtl {
 
 struct BBucket {
     int32_t inst, lanes, net0, net1, blk0;
+    int32_t deg, pin0;  // exact degree (0: read net_start) and first global pin
 };
 
 // Device pointers handed to every batched kernel (by value).
@@ -105,8 +106,12 @@ enum BCtlMode { kCtlGamma = 0, kCtlLambda = 1, kCtlProbe = 2, kCtlAlpha = 3, kCt
 enum BMoveMode { kMoveTrial = 0, kMoveProbe = 1, kMoveInitU = 2 };
 
 // ---- batched launchers (k_batch.hip)
-void bl_move(const BDev &d, int mode, int cell_blocks, hipStream_t s);
-void bl_maps(const BDev &d, int at, int bin_blocks, int sort_blocks, int dens_blocks,
+// Blocks of kBlock * kSortCellsPerThread cells (the sort's layout); kMoveTrial
+// also counts the trial positions into the sort tiles.
+void bl_move(const BDev &d, int mode, int sort_blocks, hipStream_t s);
+// Density maps of every unfinished instance; `counted`: a kMoveTrial move made
+// the sort counts at the evaluated positions.
+void bl_maps(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_blocks,
              int dens_cpt, hipStream_t s);
 void bl_wa(const BDev &d, int at, int wa_blocks, hipStream_t s);
 void bl_cell_grad(const BDev &d, int at, int cell_blocks, hipStream_t s);

@@ -39,7 +39,9 @@ constexpr int kSortCellsPerThread = 8;
 constexpr size_t kMaxStepRecords = (size_t)1 << 21;
 // Default HPWL reference of the lambda schedule: this many bin widths per net.
 constexpr double kHpwlRefPerNetBin = 0.05;
-constexpr int kMaxBuckets = 6;  // lanes per net: 2,4,8,16,32,64
+// Net classes: exact degree 2, 3, 4 (2, 4, 4 lanes per net), then degrees up
+// to 8, 16, 32 and beyond (8, 16, 32, 64 lanes).
+constexpr int kMaxBuckets = 7;
 
 // Density scatter plan: super-tile level (2^level x 2^level coarse tiles per
 // block window), window extent beyond the super tile in bins, LDS window
@@ -88,6 +90,8 @@ struct Bucket {
     int32_t lanes;  // lanes per net
     int32_t net0, net1;
     int32_t blk0;   // first block of this bucket in the WA grid
+    int32_t deg;    // exact degree of every net in the bucket (0: read net_start)
+    int32_t pin0;   // first pin of the bucket (pins of its nets are contiguous)
 };
 
 struct BucketTable {
@@ -123,7 +127,9 @@ struct Ctx;
 
 // ---- kernel launchers (defined in the .hip files) ----
 void launch_fill_maps(Ctx &c, hipStream_t s);
-void launch_sort(Ctx &c, const float2 *pos, hipStream_t s);
+// Tile-sorts the cells at pos; `counted`: the tile counts were already made by
+// a counting launch_trial at the same positions.
+void launch_sort(Ctx &c, const float2 *pos, bool counted, hipStream_t s);
 // Scatters the cells packed by the last launch_sort (same positions).
 void launch_density(Ctx &c, hipStream_t s);
 void launch_overflow(Ctx &c, hipStream_t s);
@@ -132,8 +138,10 @@ void launch_field(Ctx &c, int precision, hipStream_t s);
 void launch_cell_grad(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref,
                       float2 *gout, float2 *gw_out, float2 *f_out, hipStream_t s);
 void launch_finalize(Ctx &c, hipStream_t s);
+// Nesterov trial into v_new/u_new; `count`: also count v_new into the sort's
+// tiles (the following evaluation then sorts with counted = true).
 void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, float2 *v_new,
-                  float2 *u_new, hipStream_t s);
+                  float2 *u_new, bool count, hipStream_t s);
 void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s);
 void launch_mlp_propose(Ctx &c, hipStream_t s);
 void launch_wa_packet(const float *data, int net_size, int n_nets, int groups, float ginv,

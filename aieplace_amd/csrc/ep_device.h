@@ -371,13 +371,14 @@ __device__ inline int dens_super_tile(const int32_t *__restrict__ dtab, int ns, 
 }
 
 // Cells [c0, c1) of the sorted order in block `chunk` of super tile s.  After
-// the sort's scatter pass off[key] holds the END of key's run.
+// the sort's scatter pass off[key] holds the END of key's run (clamped to the
+// n sorted cells).
 __device__ inline void dens_cell_range(const int32_t *__restrict__ off, int s, int level, int chunk,
-                                       int K, int &c0, int &c1)
+                                       int K, int n, int &c0, int &c1)
 {
     const int key0 = s << (2 * level), key1 = (s + 1) << (2 * level);
     const int start = key0 > 0 ? off[key0 - 1] : 0;
-    const int stop = off[key1 - 1];
+    const int stop = min(off[key1 - 1], n);
     c0 = start + chunk * K;
     c1 = min(c0 + K, stop);
 }

@@ -56,35 +56,55 @@ __device__ inline const float2 *eval_pos(const BDev &d, int cur, int at)
 // ------------------------------------------------------------------ moves
 
 // kMoveTrial: u' = clamp(v - alpha g), v' = clamp(u' + coef (u' - u)) into the
-// other parity; kMoveProbe: v_-1 = clamp(v - eps g); kMoveInitU: u = v.
+// other parity, counting v' into the instance's sort tiles (the evaluation's
+// sort then skips its counting pass); kMoveProbe: v_-1 = clamp(v - eps g);
+// kMoveInitU: u = v.  Blocks take kBlock * kSortCellsPerThread cells (pre_sort).
 __global__ __launch_bounds__(kBlock) void kb_move(BDev d, int mode)
 {
-    const int i = find_inst(d.pre_cell, d.n_inst, blockIdx.x);
+    __shared__ int hist[kSortTiles];
+    const int i = find_inst(d.pre_sort, d.n_inst, blockIdx.x);
     const BState *S = d.st + i;
     if (S->done) return;
     const BInst &I = d.inst[i];
-    const int lc = (blockIdx.x - d.pre_cell[i]) * kBlock + threadIdx.x;
-    if (lc >= I.n_place) return;
-    const int c = I.cell_off + lc;
+    const bool count = mode == kMoveTrial;
+    if (count) {
+        for (int t = threadIdx.x; t < kSortTiles; t += kBlock) hist[t] = 0;
+        __syncthreads();
+    }
     const int a = S->cur;
     const GeomArgs g = inst_geom(I);
-    const float2 s = d.wh[c];
-    const float2 vv = pick<const float2 *>(d.V[0], d.V[1], a)[c];
-    if (mode == kMoveTrial) {
-        const float alpha = S->f_alpha, coef = S->f_coef;
-        const float2 uu = pick<const float2 *>(d.U[0], d.U[1], a)[c];
-        const float2 gg = pick<const float2 *>(d.G[0], d.G[1], a)[c];
-        const float2 un = clamp_cell(make_float2(vv.x - alpha * gg.x, vv.y - alpha * gg.y), s, g);
-        const float2 vn =
-            clamp_cell(make_float2(un.x + coef * (un.x - uu.x), un.y + coef * (un.y - uu.y)), s, g);
-        pick(d.U[0], d.U[1], a ^ 1)[c] = un;
-        pick(d.V[0], d.V[1], a ^ 1)[c] = vn;
-    } else if (mode == kMoveProbe) {
-        const float eps = S->f_eps;
-        const float2 gg = pick<const float2 *>(d.G[0], d.G[1], a)[c];
-        pick(d.V[0], d.V[1], a ^ 1)[c] = clamp_cell(make_float2(vv.x - eps * gg.x, vv.y - eps * gg.y), s, g);
-    } else {
-        pick(d.U[0], d.U[1], a)[c] = vv;
+    const int base = (blockIdx.x - d.pre_sort[i]) * kBlock * kSortCellsPerThread;
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k) {
+        const int lc = base + k * kBlock + threadIdx.x;
+        if (lc >= I.n_place) continue;
+        const int c = I.cell_off + lc;
+        const float2 s = d.wh[c];
+        const float2 vv = pick<const float2 *>(d.V[0], d.V[1], a)[c];
+        if (mode == kMoveTrial) {
+            const float alpha = S->f_alpha, coef = S->f_coef;
+            const float2 uu = pick<const float2 *>(d.U[0], d.U[1], a)[c];
+            const float2 gg = pick<const float2 *>(d.G[0], d.G[1], a)[c];
+            const float2 un = clamp_cell(make_float2(vv.x - alpha * gg.x, vv.y - alpha * gg.y), s, g);
+            const float2 vn =
+                clamp_cell(make_float2(un.x + coef * (un.x - uu.x), un.y + coef * (un.y - uu.y)), s, g);
+            pick(d.U[0], d.U[1], a ^ 1)[c] = un;
+            pick(d.V[0], d.V[1], a ^ 1)[c] = vn;
+            atomicAdd(&hist[cell_key(vn, s, g, I.M, I.M)], 1);
+        } else if (mode == kMoveProbe) {
+            const float eps = S->f_eps;
+            const float2 gg = pick<const float2 *>(d.G[0], d.G[1], a)[c];
+            pick(d.V[0], d.V[1], a ^ 1)[c] =
+                clamp_cell(make_float2(vv.x - eps * gg.x, vv.y - eps * gg.y), s, g);
+        } else {
+            pick(d.U[0], d.U[1], a)[c] = vv;
+        }
+    }
+    if (count) {
+        __syncthreads();
+        int32_t *cnt = d.sort_count + (size_t)i * kSortTiles;
+        for (int t = threadIdx.x; t < kSortTiles; t += kBlock)
+            if (hist[t]) atomicAdd(&cnt[t], hist[t]);
     }
 }
 
@@ -180,7 +200,10 @@ __global__ __launch_bounds__(kBlock) void kb_sort_scatter(BDev d, int at)
     float4 *sorted = d.sorted + I.cell_off;
 #pragma unroll
     for (int k = 0; k < kSortCellsPerThread; ++k)
-        if (key[k] >= 0) sorted[base_of[key[k]] + rank[k]] = cell[k];
+        if (key[k] >= 0) {
+            const int slot = base_of[key[k]] + rank[k];
+            if (slot < I.n_place) sorted[slot] = cell[k];  // counts always sum to n_place
+        }
 }
 
 // Density scatter (as k_density): the instance's blocks take up to kBlock * CPT
@@ -200,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void kb_density(BDev d)
     if (b >= dtab[ns]) return;
     const int s = dens_super_tile(dtab, ns, b);
     int c0, c1, x0, y0, nx, ny;
-    dens_cell_range(d.sort_off + (size_t)i * kSortTiles, s, level, b - dtab[s], kBlock * CPT, c0, c1);
+    dens_cell_range(d.sort_off + (size_t)i * kSortTiles, s, level, b - dtab[s], kBlock * CPT, I.n_place,
+                    c0, c1);
     super_window(s, level, I.M, I.M, I.dens_ext, x0, y0, nx, ny);
     density_window<CPT>(d.sorted + I.cell_off, c0, c1, inst_geom(I), I.M, I.M, x0, y0, nx, ny,
                         d.acc_field + I.map_off, d.acc_mov + I.map_off, d.dens_wcap);
@@ -270,8 +294,13 @@ __device__ inline void wa_tile(const BDev &d, const BBucket &bk, int blk, const 
     const bool active = net < bk.net1;
     int s = 0, e = 0;
     if (active) {
-        s = d.net_start[net];
-        e = d.net_start[net + 1];
+        if (bk.deg) {  // exact-degree class: no net_start load in the chain
+            s = bk.pin0 + (net - bk.net0) * bk.deg;
+            e = s + bk.deg;
+        } else {
+            s = d.net_start[net];
+            e = d.net_start[net + 1];
+        }
     }
     wa_net<G>(s, e, active, lane, ginv, d.pin_cell, d.pin_off, pos, d.pin_grad, hp, wa);
 }
@@ -504,16 +533,16 @@ int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 bool bl_density_cpt_supported(int cpt) { return cpt == 2 || cpt == 4 || cpt == 8; }
 
-void bl_move(const BDev &d, int mode, int cell_blocks, hipStream_t s)
+void bl_move(const BDev &d, int mode, int sort_blocks, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_move, dim3(cell_blocks), dim3(kBlock), 0, s, d, mode);
+    hipLaunchKernelGGL(kb_move, dim3(sort_blocks), dim3(kBlock), 0, s, d, mode);
 }
 
-void bl_maps(const BDev &d, int at, int bin_blocks, int sort_blocks, int dens_blocks, int dens_cpt,
-             hipStream_t s)
+void bl_maps(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_blocks,
+             int dens_cpt, hipStream_t s)
 {
     hipLaunchKernelGGL(kb_fill, dim3(bin_blocks), dim3(kBlock), 0, s, d);
-    hipLaunchKernelGGL(kb_sort_count, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
+    if (!counted) hipLaunchKernelGGL(kb_sort_count, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
     hipLaunchKernelGGL(kb_sort_scan, dim3(d.n_inst), dim3(kSortScanThreads), 0, s, d, kBlock * dens_cpt);
     hipLaunchKernelGGL(kb_sort_scatter, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
     const size_t lds = 2 * sizeof(unsigned long long) * (size_t)d.dens_wcap;

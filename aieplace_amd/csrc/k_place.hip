@@ -100,7 +100,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(const float2 *__restric
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortCellsPerThread; ++k)
-        if (key[k] >= 0) sorted[base_of[key[k]] + rank[k]] = cell[k];
+        if (key[k] >= 0) {
+            const int slot = base_of[key[k]] + rank[k];
+            if (slot < n_place) sorted[slot] = cell[k];  // counts always sum to n_place
+        }
 }
 
 // ---- density scatter: block b takes up to kBlock * CPT cells of one super
@@ -109,7 +112,8 @@ template <int CPT>
 __global__ __launch_bounds__(kBlock) void k_density(const float4 *__restrict__ sorted,
                                                     const int32_t *__restrict__ off,
                                                     const int32_t *__restrict__ dtab, int level,
-                                                    int ext, int wcap, GeomArgs g, int Mx, int My,
+                                                    int ext, int wcap, int n_place, GeomArgs g,
+                                                    int Mx, int My,
                                                     unsigned long long *__restrict__ acc_field,
                                                     unsigned long long *__restrict__ acc_mov)
 {
@@ -118,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4 *__restrict__ s
     if (b >= dtab[ns]) return;
     const int s = dens_super_tile(dtab, ns, b);
     int c0, c1, x0, y0, nx, ny;
-    dens_cell_range(off, s, level, b - dtab[s], kBlock * CPT, c0, c1);
+    dens_cell_range(off, s, level, b - dtab[s], kBlock * CPT, n_place, c0, c1);
     super_window(s, level, Mx, My, ext, x0, y0, nx, ny);
     density_window<CPT>(sorted, c0, c1, g, Mx, My, x0, y0, nx, ny, acc_field, acc_mov, wcap);
 }
@@ -191,8 +195,13 @@ __device__ inline void wa_bucket(const Bucket &bk, int blk, const int32_t *__res
     const bool active = net < bk.net1;
     int s = 0, e = 0;
     if (active) {
-        s = net_start[net];
-        e = net_start[net + 1];
+        if (bk.deg) {  // exact-degree class: no net_start load in the chain
+            s = bk.pin0 + (net - bk.net0) * bk.deg;
+            e = s + bk.deg;
+        } else {
+            s = net_start[net];
+            e = net_start[net + 1];
+        }
     }
     wa_net<G>(s, e, active, lane, ginv, pin_cell, pin_off, pos, pin_grad, hp, wa);
 }
@@ -357,21 +366,43 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
 // ------------------------------------------------------------------ Nesterov
 
 // u' = clamp(v - alpha g); v' = clamp(u' + coef (u' - u))   (ePlace Nesterov step)
+// With `count`, the new positions are also counted into the sort's coarse
+// tiles (the first pass of launch_sort, which then skips it).  Blocks take
+// kBlock * kSortCellsPerThread cells, as k_sort_count.
 __global__ __launch_bounds__(kBlock) void k_trial(const float2 *__restrict__ v,
                                                   const float2 *__restrict__ u,
                                                   const float2 *__restrict__ gr,
                                                   const float2 *__restrict__ wh, int n_place,
-                                                  GeomArgs g, const DevScalars *__restrict__ sc,
+                                                  GeomArgs g, int Mx, int My,
+                                                  const DevScalars *__restrict__ sc,
                                                   float2 *__restrict__ v_new,
-                                                  float2 *__restrict__ u_new)
+                                                  float2 *__restrict__ u_new,
+                                                  int32_t *__restrict__ count)
 {
+    __shared__ int hist[kSortTiles];
+    if (count) {
+        for (int t = threadIdx.x; t < kSortTiles; t += kBlock) hist[t] = 0;
+        __syncthreads();
+    }
     const float alpha = sc->alpha, coef = sc->coef;
-    for (int c = blockIdx.x * kBlock + threadIdx.x; c < n_place; c += gridDim.x * kBlock) {
-        const float2 s = wh[c], vv = v[c], uu = u[c], gg = gr[c];
-        float2 un = clamp_cell(make_float2(vv.x - alpha * gg.x, vv.y - alpha * gg.y), s, g);
-        float2 vn = clamp_cell(make_float2(un.x + coef * (un.x - uu.x), un.y + coef * (un.y - uu.y)), s, g);
-        u_new[c] = un;
-        v_new[c] = vn;
+    const int base = blockIdx.x * kBlock * kSortCellsPerThread;
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k) {
+        const int c = base + k * kBlock + threadIdx.x;
+        if (c < n_place) {
+            const float2 s = wh[c], vv = v[c], uu = u[c], gg = gr[c];
+            const float2 un = clamp_cell(make_float2(vv.x - alpha * gg.x, vv.y - alpha * gg.y), s, g);
+            const float2 vn =
+                clamp_cell(make_float2(un.x + coef * (un.x - uu.x), un.y + coef * (un.y - uu.y)), s, g);
+            u_new[c] = un;
+            v_new[c] = vn;
+            if (count) atomicAdd(&hist[cell_key(vn, s, g, Mx, My)], 1);
+        }
+    }
+    if (count) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < kSortTiles; t += kBlock)
+            if (hist[t]) atomicAdd(&count[t], hist[t]);
     }
 }
 
@@ -432,11 +463,12 @@ void launch_fill_maps(Ctx &c, hipStream_t s)
                        c.acc_field, c.acc_mov, nb);
 }
 
-void launch_sort(Ctx &c, const float2 *pos, hipStream_t s)
+void launch_sort(Ctx &c, const float2 *pos, bool counted, hipStream_t s)
 {
     const int blocks = grid_for(c.n_place, kBlock * kSortCellsPerThread, INT_MAX);
-    hipLaunchKernelGGL(k_sort_count, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
-                       geom_args(c), c.Mx, c.My, c.sort_count);
+    if (!counted)
+        hipLaunchKernelGGL(k_sort_count, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
+                           geom_args(c), c.Mx, c.My, c.sort_count);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortScanThreads), 0, s, c.sort_count, c.sort_off,
                        c.dens_tab, c.dens.level, kBlock * c.dens.cpt);
     hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
@@ -450,7 +482,8 @@ void launch_density(Ctx &c, hipStream_t s)
     const size_t lds = dens_lds_bytes(dp);
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, s, c.sorted, c.sort_off, c.dens_tab,
-                           dp.level, dp.ext, dp.wcap, geom_args(c), c.Mx, c.My, c.acc_field, c.acc_mov);
+                           dp.level, dp.ext, dp.wcap, c.n_place, geom_args(c), c.Mx, c.My,
+                           c.acc_field, c.acc_mov);
     };
     if (dp.cpt == 2) go(k_density<2>);
     else if (dp.cpt == 8) go(k_density<8>);
@@ -494,10 +527,12 @@ void launch_finalize(Ctx &c, hipStream_t s)
 }
 
 void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, float2 *v_new,
-                  float2 *u_new, hipStream_t s)
+                  float2 *u_new, bool count, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_trial, dim3(grid_for(c.n_place, kBlock, 8192)), dim3(kBlock), 0, s, v, u,
-                       g, c.wh, c.n_place, geom_args(c), c.d_scal, v_new, u_new);
+    const int blocks = grid_for(c.n_place, kBlock * kSortCellsPerThread, INT_MAX);
+    hipLaunchKernelGGL(k_trial, dim3(blocks), dim3(kBlock), 0, s, v, u, g, c.wh, c.n_place,
+                       geom_args(c), c.Mx, c.My, c.d_scal, v_new, u_new,
+                       count ? c.sort_count : nullptr);
 }
 
 void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s)
