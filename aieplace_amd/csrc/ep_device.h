@@ -166,26 +166,53 @@ __device__ inline int cell_key(float2 p, float2 s, const GeomArgs &g, int Mx, in
     return morton6((i * kSortSide) / Mx, (j * kSortSide) / My);
 }
 
+// All-reduce over aligned groups of G lanes.  Inside a 16-lane row the steps
+// are DPP lane moves folded into the ALU op (no LDS crossbar): xor 1 and xor 2
+// by quad permutes, then the half-row and row mirrors, which pair every lane
+// with one of the other quad / half-row once those are uniform.  Groups of 32
+// and 64 finish with cross-row shuffles.
+template <int CTRL>
+__device__ inline float dpp_mov(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+template <int G, class Op>
+__device__ inline float grp_reduce(float v, Op op)
+{
+    if (G >= 2) v = op(v, dpp_mov<0xB1>(v));   // quad_perm [1,0,3,2]: lane ^ 1
+    if (G >= 4) v = op(v, dpp_mov<0x4E>(v));   // quad_perm [2,3,0,1]: lane ^ 2
+    if (G >= 8) v = op(v, dpp_mov<0x141>(v));  // row_half_mirror: lane i <-> 7 - i
+    if (G >= 16) v = op(v, dpp_mov<0x140>(v)); // row_mirror: lane i <-> 15 - i
+    if (G >= 32) v = op(v, __shfl_xor(v, 16, 64));
+    if (G >= 64) v = op(v, __shfl_xor(v, 32, 64));
+    return v;
+}
+
+struct OpMax {
+    __device__ float operator()(float a, float b) const { return fmaxf(a, b); }
+};
+struct OpMin {
+    __device__ float operator()(float a, float b) const { return fminf(a, b); }
+};
+struct OpSum {
+    __device__ float operator()(float a, float b) const { return a + b; }
+};
+
 template <int G>
 __device__ inline float grp_max(float v)
 {
-#pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, G));
-    return v;
+    return grp_reduce<G>(v, OpMax());
 }
 template <int G>
 __device__ inline float grp_min(float v)
 {
-#pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m, G));
-    return v;
+    return grp_reduce<G>(v, OpMin());
 }
 template <int G>
 __device__ inline float grp_sum(float v)
 {
-#pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
-    return v;
+    return grp_reduce<G>(v, OpSum());
 }
 
 // One net per group of G lanes.  Shifted WA form (algebraically equal to
@@ -224,12 +251,15 @@ __device__ inline void wa_net(int s, int e, bool active, int lane, float ginv,
         const float byp = grp_sum<G>(ayp), bym = grp_sum<G>(aym);
         const float cxp = grp_sum<G>(dxp * axp), cxm = grp_sum<G>(dxm * axm);
         const float cyp = grp_sum<G>(dyp * ayp), cym = grp_sum<G>(dym * aym);
-        const float mxp = cxp / bxp, mxm = cxm / bxm, myp = cyp / byp, mym = cym / bym;
+        // one hardware reciprocal (1 ulp) per sum, shared by the lane's quotients
+        const float rxp = __builtin_amdgcn_rcpf(bxp), rxm = __builtin_amdgcn_rcpf(bxm);
+        const float ryp = __builtin_amdgcn_rcpf(byp), rym = __builtin_amdgcn_rcpf(bym);
+        const float mxp = cxp * rxp, mxm = cxm * rxm, myp = cyp * ryp, mym = cym * rym;
         if (valid) {
-            const float gx = axp / bxp * (1.f + (dxp - mxp) * ginv) -
-                             axm / bxm * (1.f - (dxm - mxm) * ginv);
-            const float gy = ayp / byp * (1.f + (dyp - myp) * ginv) -
-                             aym / bym * (1.f - (dym - mym) * ginv);
+            const float gx = axp * rxp * (1.f + (dxp - mxp) * ginv) -
+                             axm * rxm * (1.f - (dxm - mxm) * ginv);
+            const float gy = ayp * ryp * (1.f + (dyp - myp) * ginv) -
+                             aym * rym * (1.f - (dym - mym) * ginv);
             pin_grad[s + lane] = make_float2(gx, gy);
         }
         if (active && lane == 0) {
@@ -259,17 +289,19 @@ __device__ inline void wa_net(int s, int e, bool active, int lane, float ginv,
         }
         bxp = grp_sum<G>(bxp); bxm = grp_sum<G>(bxm); byp = grp_sum<G>(byp); bym = grp_sum<G>(bym);
         cxp = grp_sum<G>(cxp); cxm = grp_sum<G>(cxm); cyp = grp_sum<G>(cyp); cym = grp_sum<G>(cym);
-        const float mxp = cxp / bxp, mxm = cxm / bxm, myp = cyp / byp, mym = cym / bym;
+        const float rxp = __builtin_amdgcn_rcpf(bxp), rxm = __builtin_amdgcn_rcpf(bxm);
+        const float ryp = __builtin_amdgcn_rcpf(byp), rym = __builtin_amdgcn_rcpf(bym);
+        const float mxp = cxp * rxp, mxm = cxm * rxm, myp = cyp * ryp, mym = cym * rym;
         for (int p = s + lane; p < e; p += G) {
             const float2 q = pos[pin_cell[p]], o = pin_off[p];
             const float dxp = q.x + o.x - xmx, dxm = q.x + o.x - xmn;
             const float dyp = q.y + o.y - ymx, dym = q.y + o.y - ymn;
             const float axp = __expf(dxp * ginv), axm = __expf(-dxm * ginv);
             const float ayp = __expf(dyp * ginv), aym = __expf(-dym * ginv);
-            const float gx = axp / bxp * (1.f + (dxp - mxp) * ginv) -
-                             axm / bxm * (1.f - (dxm - mxm) * ginv);
-            const float gy = ayp / byp * (1.f + (dyp - myp) * ginv) -
-                             aym / bym * (1.f - (dym - mym) * ginv);
+            const float gx = axp * rxp * (1.f + (dxp - mxp) * ginv) -
+                             axm * rxm * (1.f - (dxm - mxm) * ginv);
+            const float gy = ayp * ryp * (1.f + (dyp - myp) * ginv) -
+                             aym * rym * (1.f - (dym - mym) * ginv);
             pin_grad[p] = make_float2(gx, gy);
         }
         if (lane == 0) {
