@@ -112,16 +112,43 @@ __device__ inline float ovl(float t0, float t1, int i)
     return fminf(t1, (float)(i + 1)) - fmaxf(t0, (float)i);
 }
 
+// DPP lane move of a double (two 32-bit moves).
+template <int CTRL>
+__device__ inline double dpp_mov_d(double v)
+{
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ inline double readlane_d(double v, int lane)
+{
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Wave-wide sum / max (every lane gets the result), in a fixed order: DPP
+// all-reduce inside each 16-lane row (as grp_reduce), then the four row
+// totals in row order.
 __device__ inline double wave_sum(double v)
 {
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
+    v += dpp_mov_d<0xB1>(v);
+    v += dpp_mov_d<0x4E>(v);
+    v += dpp_mov_d<0x141>(v);
+    v += dpp_mov_d<0x140>(v);
+    return ((readlane_d(v, 0) + readlane_d(v, 16)) + readlane_d(v, 32)) + readlane_d(v, 48);
 }
 
 __device__ inline double wave_max(double v)
 {
-    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
-    return v;
+    v = fmax(v, dpp_mov_d<0xB1>(v));
+    v = fmax(v, dpp_mov_d<0x4E>(v));
+    v = fmax(v, dpp_mov_d<0x141>(v));
+    v = fmax(v, dpp_mov_d<0x140>(v));
+    return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
 }
 
 // Block-wide sum (and max) of NV doubles; thread 0 writes out[k].
