@@ -171,7 +171,8 @@ __device__ inline float sym_store(const GemmArgs &a, int z, int h0, int n0, cons
 
 // ---------------------------------------------------------------- FP32
 
-constexpr int SK32 = 16;  // half-k per step
+constexpr int SK32 = 32;           // half-k per step
+constexpr int P32 = SK32 / 16;     // load pieces per thread and step
 
 template <bool INV, int EPI>
 __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
@@ -184,47 +185,57 @@ __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
     __shared__ float sB[2][SK32][SBN + 4];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int eo = wv >> 1, wm = (wv & 1) * 32;
-    const int ar = t >> 2, ak = (t & 3) * 4;  // A: 64 rows x 16 k, 4 per thread and half
-    const int br = t >> 3, bk = (t & 7) * 2;  // B: 32 rows x 16 half-k, 2 per thread and plane
+    const int ar = t >> 2, ak = (t & 3) * 4;  // A: 64 rows x 16 k per piece, 4 per thread and half
+    const int br = t >> 3, bk = (t & 7) * 2;  // B: 32 rows x 16 half-k per piece, 2 per thread and plane
     const bool a_ok = h0 + ar < Mh, b_ok = n0 + br < N;
     const float bias = (!INV && a.bbias && b_ok) ? *a.bbias : 0.0f;
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-    float4 ae, ao;
-    float2 b0, b1;
+    float4 ae[P32], ao[P32];
+    float2 b0[P32], b1[P32];
     auto load = [&](int k0) {
-        ae = ao = make_float4(0, 0, 0, 0);
-        b0 = b1 = make_float2(0, 0);
-        if (a_ok && k0 + ak < Kh) {
-            ae = *reinterpret_cast<const float4 *>(&Ae[(size_t)(h0 + ar) * Kh + k0 + ak]);
-            ao = *reinterpret_cast<const float4 *>(&Ao[(size_t)(h0 + ar) * Kh + k0 + ak]);
-        }
-        if (b_ok && k0 + bk < Kh) {
-            const float *row = B + (size_t)(n0 + br) * K;
-            if (INV) {
-                const float4 v = *reinterpret_cast<const float4 *>(&row[2 * (k0 + bk)]);
-                b0 = make_float2(v.x, v.z);  // b[2k], b[2k+2]
-                b1 = make_float2(v.y, v.w);  // b[2k+1], b[2k+3]
-            } else {
-                b0 = *reinterpret_cast<const float2 *>(&row[k0 + bk]);          // b[k], b[k+1]
-                b1 = *reinterpret_cast<const float2 *>(&row[K - 2 - k0 - bk]);  // b[K-2-k], b[K-1-k]
+#pragma unroll
+        for (int q = 0; q < P32; ++q) {
+            const int ka = k0 + q * 16 + ak, kb = k0 + q * 16 + bk;
+            ae[q] = ao[q] = make_float4(0, 0, 0, 0);
+            b0[q] = b1[q] = make_float2(0, 0);
+            if (a_ok && ka < Kh) {
+                ae[q] = *reinterpret_cast<const float4 *>(&Ae[(size_t)(h0 + ar) * Kh + ka]);
+                ao[q] = *reinterpret_cast<const float4 *>(&Ao[(size_t)(h0 + ar) * Kh + ka]);
+            }
+            if (b_ok && kb < Kh) {
+                const float *row = B + (size_t)(n0 + br) * K;
+                if (INV) {
+                    const float4 v = *reinterpret_cast<const float4 *>(&row[2 * kb]);
+                    b0[q] = make_float2(v.x, v.z);  // b[2k], b[2k+2]
+                    b1[q] = make_float2(v.y, v.w);  // b[2k+1], b[2k+3]
+                } else {
+                    b0[q] = *reinterpret_cast<const float2 *>(&row[kb]);          // b[k], b[k+1]
+                    b1[q] = *reinterpret_cast<const float2 *>(&row[K - 2 - kb]);  // b[K-2-k], b[K-1-k]
+                }
             }
         }
     };
     load(0);
     for (int k0 = 0; k0 < Kh; k0 += SK32) {
         __syncthreads();
-        sA[0][ak + 0][ar] = ae.x; sA[0][ak + 1][ar] = ae.y; sA[0][ak + 2][ar] = ae.z; sA[0][ak + 3][ar] = ae.w;
-        sA[1][ak + 0][ar] = ao.x; sA[1][ak + 1][ar] = ao.y; sA[1][ak + 2][ar] = ao.z; sA[1][ak + 3][ar] = ao.w;
-        if (INV) {
-            sB[0][bk][br] = b0.x; sB[0][bk + 1][br] = b0.y;
-            sB[1][bk][br] = b1.x; sB[1][bk + 1][br] = b1.y;
-        } else {
-            const float d0 = b0.x - bias, d1 = b0.y - bias;  // b[k], b[k+1]
-            const float m0 = b1.y - bias, m1 = b1.x - bias;  // b[K-1-k], b[K-2-k]
-            sB[0][bk][br] = d0 + m0; sB[0][bk + 1][br] = d1 + m1;
-            sB[1][bk][br] = d0 - m0; sB[1][bk + 1][br] = d1 - m1;
+#pragma unroll
+        for (int q = 0; q < P32; ++q) {
+            const int ka = q * 16 + ak, kb = q * 16 + bk;
+            sA[0][ka + 0][ar] = ae[q].x; sA[0][ka + 1][ar] = ae[q].y;
+            sA[0][ka + 2][ar] = ae[q].z; sA[0][ka + 3][ar] = ae[q].w;
+            sA[1][ka + 0][ar] = ao[q].x; sA[1][ka + 1][ar] = ao[q].y;
+            sA[1][ka + 2][ar] = ao[q].z; sA[1][ka + 3][ar] = ao[q].w;
+            if (INV) {
+                sB[0][kb][br] = b0[q].x; sB[0][kb + 1][br] = b0[q].y;
+                sB[1][kb][br] = b1[q].x; sB[1][kb + 1][br] = b1[q].y;
+            } else {
+                const float d0 = b0[q].x - bias, d1 = b0[q].y - bias;  // b[k], b[k+1]
+                const float m0 = b1[q].y - bias, m1 = b1[q].x - bias;  // b[K-1-k], b[K-2-k]
+                sB[0][kb][br] = d0 + m0; sB[0][kb + 1][br] = d1 + m1;
+                sB[1][kb][br] = d0 - m0; sB[1][kb + 1][br] = d1 - m1;
+            }
         }
         __syncthreads();
         if (k0 + SK32 < Kh) load(k0 + SK32);  // next tiles in flight during the MFMAs
@@ -246,7 +257,8 @@ __global__ __launch_bounds__(256) void k_sym_f32(GemmArgs a)
 
 // ---------------------------------------------------------------- split 16-bit
 
-constexpr int SK16 = 32;          // half-k per step (two 32x32x16 MFMA sub-steps)
+constexpr int SK16 = 64;          // half-k per step (four 32x32x16 MFMA sub-steps)
+constexpr int P16 = SK16 / 32;    // load pieces per thread and step
 constexpr int SLDK = SK16 + 8;    // padded LDS row (16-bit elements)
 
 __device__ inline uint16_t bf16_rne(float x)
@@ -307,8 +319,8 @@ __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
     __shared__ __attribute__((aligned(16))) uint16_t sBl[2][SBN][SLDK];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int eo = wv >> 1, wm = (wv & 1) * 32;
-    const int ar = t >> 2, ak = (t & 3) * 8;  // A: 64 rows x 32 half-k, 8 per thread and plane
-    const int br = t >> 3, bk = (t & 7) * 4;  // B: 32 rows x 32 half-k, 4 per thread and plane
+    const int ar = t >> 2, ak = (t & 3) * 8;  // A: 64 rows x 32 half-k per piece, 8 per thread and plane
+    const int br = t >> 3, bk = (t & 7) * 4;  // B: 32 rows x 32 half-k per piece, 4 per thread and plane
     const bool a_ok = h0 + ar < Mh, b_ok = n0 + br < N;
     // FP16: power-of-two scale of the data operand (exact; undone in the
     // epilogue).  The forward butterfly adds two values: one binade lower.
@@ -322,61 +334,67 @@ __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-    uint4 aeh, ael, aoh, aol;
-    float4 b0, b1;
+    uint4 aeh[P16], ael[P16], aoh[P16], aol[P16];
+    float4 b0[P16], b1[P16];
     auto load = [&](int k0) {
-        aeh = ael = aoh = aol = make_uint4(0, 0, 0, 0);
-        b0 = b1 = make_float4(0, 0, 0, 0);
-        if (a_ok && k0 + ak < Kh) {
-            const size_t o = (size_t)(h0 + ar) * Kh + k0 + ak;
-            aeh = *reinterpret_cast<const uint4 *>(&Aeh[o]);
-            ael = *reinterpret_cast<const uint4 *>(&Ael[o]);
-            aoh = *reinterpret_cast<const uint4 *>(&Aoh[o]);
-            aol = *reinterpret_cast<const uint4 *>(&Aol[o]);
-        }
-        if (b_ok && k0 + bk < Kh) {
-            const float *row = B + (size_t)(n0 + br) * K;
-            if (INV) {
-                const float4 v0 = *reinterpret_cast<const float4 *>(&row[2 * (k0 + bk)]);
-                const float4 v1 = *reinterpret_cast<const float4 *>(&row[2 * (k0 + bk) + 4]);
-                b0 = make_float4(v0.x, v0.z, v1.x, v1.z);  // b[2k .. 2k+6 even]
-                b1 = make_float4(v0.y, v0.w, v1.y, v1.w);  // odd
-            } else {
-                b0 = *reinterpret_cast<const float4 *>(&row[k0 + bk]);          // b[k .. k+3]
-                b1 = *reinterpret_cast<const float4 *>(&row[K - 4 - k0 - bk]);  // b[K-4-k .. K-1-k]
+#pragma unroll
+        for (int q = 0; q < P16; ++q) {
+            const int ka = k0 + q * 32 + ak, kb = k0 + q * 32 + bk;
+            aeh[q] = ael[q] = aoh[q] = aol[q] = make_uint4(0, 0, 0, 0);
+            b0[q] = b1[q] = make_float4(0, 0, 0, 0);
+            if (a_ok && ka < Kh) {
+                const size_t o = (size_t)(h0 + ar) * Kh + ka;
+                aeh[q] = *reinterpret_cast<const uint4 *>(&Aeh[o]);
+                ael[q] = *reinterpret_cast<const uint4 *>(&Ael[o]);
+                aoh[q] = *reinterpret_cast<const uint4 *>(&Aoh[o]);
+                aol[q] = *reinterpret_cast<const uint4 *>(&Aol[o]);
+            }
+            if (b_ok && kb < Kh) {
+                const float *row = B + (size_t)(n0 + br) * K;
+                if (INV) {
+                    const float4 v0 = *reinterpret_cast<const float4 *>(&row[2 * kb]);
+                    const float4 v1 = *reinterpret_cast<const float4 *>(&row[2 * kb + 4]);
+                    b0[q] = make_float4(v0.x, v0.z, v1.x, v1.z);  // b[2k .. 2k+6 even]
+                    b1[q] = make_float4(v0.y, v0.w, v1.y, v1.w);  // odd
+                } else {
+                    b0[q] = *reinterpret_cast<const float4 *>(&row[kb]);          // b[k .. k+3]
+                    b1[q] = *reinterpret_cast<const float4 *>(&row[K - 4 - kb]);  // b[K-4-k .. K-1-k]
+                }
             }
         }
     };
     load(0);
     for (int k0 = 0; k0 < Kh; k0 += SK16) {
         __syncthreads();
-        *reinterpret_cast<uint4 *>(&sAh[0][ar][ak]) = aeh;
-        *reinterpret_cast<uint4 *>(&sAl[0][ar][ak]) = ael;
-        *reinterpret_cast<uint4 *>(&sAh[1][ar][ak]) = aoh;
-        *reinterpret_cast<uint4 *>(&sAl[1][ar][ak]) = aol;
-        {
-            float p[4], q[4];
+#pragma unroll
+        for (int q = 0; q < P16; ++q) {
+            const int ka = q * 32 + ak, kb = q * 32 + bk;
+            *reinterpret_cast<uint4 *>(&sAh[0][ar][ka]) = aeh[q];
+            *reinterpret_cast<uint4 *>(&sAl[0][ar][ka]) = ael[q];
+            *reinterpret_cast<uint4 *>(&sAh[1][ar][ka]) = aoh[q];
+            *reinterpret_cast<uint4 *>(&sAl[1][ar][ka]) = aol[q];
+            float p[4], m[4];
             if (INV) {
-                p[0] = b0.x; p[1] = b0.y; p[2] = b0.z; p[3] = b0.w;
-                q[0] = b1.x; q[1] = b1.y; q[2] = b1.z; q[3] = b1.w;
+                p[0] = b0[q].x; p[1] = b0[q].y; p[2] = b0[q].z; p[3] = b0[q].w;
+                m[0] = b1[q].x; m[1] = b1[q].y; m[2] = b1[q].z; m[3] = b1[q].w;
             } else {
-                const float d[4] = {b0.x - bias, b0.y - bias, b0.z - bias, b0.w - bias};
-                const float mi[4] = {b1.w - bias, b1.z - bias, b1.y - bias, b1.x - bias};  // b[K-1-k-j]
+                const float d[4] = {b0[q].x - bias, b0[q].y - bias, b0[q].z - bias, b0[q].w - bias};
+                const float mi[4] = {b1[q].w - bias, b1[q].z - bias, b1[q].y - bias, b1[q].x - bias};
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     p[j] = d[j] + mi[j];
-                    q[j] = d[j] - mi[j];
+                    m[j] = d[j] - mi[j];
                 }
             }
             if (F16) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     p[j] *= bscale;
-                    q[j] *= bscale;
+                    m[j] *= bscale;
                 }
             }
-            split_store4<F16>(p, &sBh[0][br][bk], &sBl[0][br][bk]);
-            split_store4<F16>(q, &sBh[1][br][bk], &sBl[1][br][bk]);
+            split_store4<F16>(p, &sBh[0][br][kb], &sBl[0][br][kb]);
+            split_store4<F16>(m, &sBh[1][br][kb], &sBl[1][br][kb]);
         }
         __syncthreads();
         if (k0 + SK16 < Kh) load(k0 + SK16);
