@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/eplace_mlp.h"
+#include "ep_sched.h"
 #include "ep_ctx.h"
 #include "ep_layout.h"
 
@@ -106,6 +107,18 @@ ep_status sync(Ctx &c)
 {
     EP_HIP(hipStreamSynchronize(c.stream));
     return EP_OK;
+}
+
+// Destroys the captured graphs (they bake the precision and the proposal kernel).
+void drop_graphs(Ctx &c)
+{
+    for (int i = 0; i < 2; ++i) {
+        if (c.graph_exec[i]) (void)hipGraphExecDestroy(c.graph_exec[i]);
+        c.graph_exec[i] = nullptr;
+        c.graph_built[i] = false;
+    }
+    if (c.ctl_exec) (void)hipGraphExecDestroy(c.ctl_exec);
+    c.ctl_exec = nullptr;
 }
 
 // Trial step from V[cur]/U[cur]/G[cur] into the other buffers, then evaluate.
@@ -224,16 +237,18 @@ static void destroy_ctx(ep_ctx *c)
     if (!c) return;
     for (int i = 0; i < 2; ++i)
         if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
+    if (c->ctl_exec) (void)hipGraphExecDestroy(c->ctl_exec);
     void *ptrs[] = {c->wh, c->net_start, c->pin_cell, c->pin_off, c->pin_grad, c->cell_pin_start,
                     c->cell_pins, c->V[0], c->V[1], c->U[0], c->U[1], c->G[0], c->G[1],
                     c->map_field, c->map_mov, c->map_fix, c->acc_field, c->acc_mov, c->ex, c->ey, c->tA, c->tB, c->part,
                     c->sorted, c->sort_count, c->sort_off, c->dens_tab,
-                    c->d_scal, c->d_res, c->d_mlp_w};
+                    c->d_scal, c->d_res, c->d_mlp_w, c->d_ctl, c->d_log, c->d_rec};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     field_free(*c);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->h_res) (void)hipHostFree(c->h_res);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -321,7 +336,7 @@ static void fixed_map(const HostLayout &L, std::vector<float> &out)
 double ep::gamma_at(const ep_params &p, double bw, double tau)
 {
     double g = (double)p.gamma_bins * bw;
-    if (p.gamma_schedule) g *= std::pow(10.0, (tau - 0.1) * 20.0 / 9.0 - 1.0);
+    if (p.gamma_schedule) g *= sched_exp10((tau - 0.1) * 20.0 / 9.0 - 1.0);
     return g;
 }
 
@@ -558,6 +573,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     }
     std::memset(c->h_scal, 0, sizeof(DevScalars));
     std::memset(c->h_res, 0, sizeof(DevResult));
+    EP_TRY(upload(c->d_scal, c->h_scal, 1));  // sel = halt = 0 before the first push
     EP_TRY(upload(c->wh, L.wh.data(), L.wh.size()));
     EP_TRY(upload(c->net_start, L.net_start.data(), L.net_start.size()));
     EP_TRY(upload(c->pin_cell, L.pin_cell.data(), L.pin_cell.size()));
@@ -851,11 +867,7 @@ extern "C" ep_status ep_set_precision(ep_ctx *c, int32_t precision)
     if (pst != EP_OK) return pst;
     if (precision != c->prm.precision) {
         c->prm.precision = precision;
-        for (int i = 0; i < 2; ++i) {
-            if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
-            c->graph_exec[i] = nullptr;
-            c->graph_built[i] = false;
-        }
+        drop_graphs(*c);
     }
     return EP_OK;
 }
@@ -922,7 +934,7 @@ extern "C" ep_status ep_init_optimizer(ep_ctx *c)
 // Step-model features of the coming iteration (layout in include/eplace_mlp.h).
 static void step_features(const ep_ctx *c, double coef, float *f)
 {
-    auto lg = [](double v) { return (float)std::log10(std::max(v, 1e-30)); };
+    auto lg = [](double v) { return (float)sched_log10(std::max(v, 1e-30)); };
     f[0] = lg(c->alpha);
     f[1] = lg(c->h_alpha1);
     f[2] = lg(c->h_alpha2);
@@ -998,7 +1010,7 @@ extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
     // record (features, accepted step) for training the step model
     if (c->rec_target.size() < kMaxStepRecords) {
         c->rec_feat.insert(c->rec_feat.end(), feat, feat + EP_MLP_IN);
-        c->rec_target.push_back((float)std::log10(std::max(used, 1e-30)));
+        c->rec_target.push_back((float)sched_log10(std::max(used, 1e-30)));
     }
     // accept: the trial buffers become the current ones
     c->cur ^= 1;
@@ -1032,12 +1044,167 @@ extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
     }
     // lambda (mu schedule) and gamma updates for the next iteration
     const double dh = hp - c->prev_hpwl;
-    double mu = dh < 0 ? c->prm.mu_max : std::pow((double)c->prm.mu_max, 1.0 - dh / c->prm.hpwl_ref);
+    double mu = dh < 0 ? c->prm.mu_max : sched_pow((double)c->prm.mu_max, 1.0 - dh / c->prm.hpwl_ref);
     mu = std::min<double>(std::max<double>(mu, c->prm.mu_min), c->prm.mu_max);
     c->lambda *= mu;
     c->prev_hpwl = hp;
     c->gamma = gamma_of(*c, tau);
     c->iteration++;
+    return EP_OK;
+}
+
+// ---------------------------------------------------------------- device-controlled run
+//
+// ep_run with graphs: one captured graph = model proposal, trial, evaluation
+// and k_step_ctl (the ep_step decision on the device).  The kernels take the
+// current and trial buffers from Pair arguments indexed by the device's `cur`,
+// so the same graph serves every trial; the host replays it kCtlReplays times
+// per synchronisation and reads the control state back.  After the stop the
+// remaining replays of a batch return at once (halt flag).
+
+constexpr int kCtlReplays = 6;
+
+static CtlParams ctl_params(const ep_ctx *c)
+{
+    CtlParams p{};
+    p.mu_min = c->prm.mu_min;
+    p.mu_max = c->prm.mu_max;
+    p.hpwl_ref = c->prm.hpwl_ref;
+    p.stop_overflow = c->prm.stop_overflow;
+    p.gamma_base = (double)c->prm.gamma_bins * (double)c->bw;  // gamma_at's base
+    p.gamma_schedule = c->prm.gamma_schedule;
+    p.max_iters = c->prm.max_iters;
+    p.max_backtrack = c->prm.max_backtrack;
+    p.learned = (c->prm.learned_step && c->mlp_loaded) ? 1 : 0;
+    p.log_cap = c->ctl_cap;
+    p.rec_cap = c->ctl_cap;
+    p.bw = c->bw;
+    return p;
+}
+
+static ep_status ensure_ctl(ep_ctx *c)
+{
+    const int cap = std::max(1, c->prm.max_iters);
+    if (c->ctl_cap == cap) return EP_OK;
+    if (c->ctl_exec) (void)hipGraphExecDestroy(c->ctl_exec);
+    c->ctl_exec = nullptr;
+    if (c->d_log) (void)hipFree(c->d_log);
+    if (c->d_rec) (void)hipFree(c->d_rec);
+    c->d_log = nullptr;
+    c->d_rec = nullptr;
+    c->ctl_cap = 0;
+    ep_status st;
+    if (!c->d_ctl && (st = dalloc(&c->d_ctl, 1)) != EP_OK) return st;
+    if (!c->h_ctl && hipHostMalloc((void **)&c->h_ctl, sizeof(DevCtl)) != hipSuccess)
+        return fail(EP_ENOMEM, "ep_run: pinned control state allocation failed");
+    if ((st = dalloc(&c->d_log, (size_t)cap)) != EP_OK) return st;
+    if ((st = dalloc(&c->d_rec, (size_t)cap * 17)) != EP_OK) return st;
+    c->ctl_cap = cap;
+    return EP_OK;
+}
+
+static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
+{
+    Ctx &x = *c;
+    hipStream_t s = x.stream;
+    const Pair<const float2> vc{{x.V[0], x.V[1]}}, uc{{x.U[0], x.U[1]}}, gc{{x.G[0], x.G[1]}};
+    const Pair<float2> vn{{x.V[1], x.V[0]}}, un{{x.U[1], x.U[0]}}, gn{{x.G[1], x.G[0]}};
+    const Pair<const float2> vt{{x.V[1], x.V[0]}};
+    hipGraph_t g = nullptr;
+    EP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    launch_mlp_propose(x, s);
+    launch_trial(x, vc, uc, gc, vn, un, true, s);
+    launch_fill_maps(x, s);
+    launch_sort(x, vt, true, s);
+    launch_density(x, s);
+    launch_overflow(x, s);
+    launch_wa(x, vt, s);
+    launch_field(x, x.prm.precision, s);
+    launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s);
+    launch_finalize(x, s);
+    launch_step_ctl(x, p, 1, s);
+    EP_HIP(hipStreamEndCapture(s, &g));
+    hipError_t e = hipGraphInstantiate(&c->ctl_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->ctl_exec = nullptr;
+        return fail(EP_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return EP_OK;
+}
+
+// Host optimiser state <-> DevCtl.
+static void ctl_upload_state(const ep_ctx *c, DevCtl &S)
+{
+    std::memset(&S, 0, sizeof(S));
+    S.alpha = c->alpha; S.ak = c->ak; S.lambda = c->lambda; S.gamma = c->gamma;
+    S.prev_hpwl = c->prev_hpwl; S.last_hpwl = c->last_hpwl; S.last_tau = c->last_tau;
+    S.last_wa = c->last_wa;
+    S.h_alpha1 = c->h_alpha1; S.h_alpha2 = c->h_alpha2; S.h_prev_tau = c->h_prev_tau;
+    S.h_prev_hpwl = c->h_prev_hpwl; S.h_ratio = c->h_ratio; S.h_gnorm = c->h_gnorm;
+    S.h_dv = c->h_dv; S.h_dg = c->h_dg; S.h_sum_ratio = c->h_sum_ratio;
+    S.evals = c->evals; S.iteration = c->iteration; S.h_evals = c->h_evals;
+    S.learned_proposals = c->learned_proposals; S.learned_accepted = c->learned_accepted;
+    S.cur = c->cur;
+}
+
+static void ctl_download_state(ep_ctx *c, const DevCtl &S)
+{
+    c->alpha = S.alpha; c->ak = S.ak; c->lambda = S.lambda; c->gamma = S.gamma;
+    c->prev_hpwl = S.prev_hpwl; c->last_hpwl = S.last_hpwl; c->last_tau = S.last_tau;
+    c->last_wa = S.last_wa;
+    c->h_alpha1 = S.h_alpha1; c->h_alpha2 = S.h_alpha2; c->h_prev_tau = S.h_prev_tau;
+    c->h_prev_hpwl = S.h_prev_hpwl; c->h_ratio = S.h_ratio; c->h_gnorm = S.h_gnorm;
+    c->h_dv = S.h_dv; c->h_dg = S.h_dg; c->h_sum_ratio = S.h_sum_ratio;
+    c->evals = S.evals; c->iteration = S.iteration; c->h_evals = S.h_evals;
+    c->learned_proposals = S.learned_proposals; c->learned_accepted = S.learned_accepted;
+    c->cur = S.cur;
+}
+
+// Iterations of ep_run under device control; the caller has set up the optimiser.
+static ep_status run_device(ep_ctx *c, ep_step_info *log, int32_t log_cap, int *n_iter)
+{
+    *n_iter = 0;
+    ep_status st = ensure_ctl(c);
+    if (st != EP_OK) return st;
+    const CtlParams p = ctl_params(c);
+    if (!c->ctl_exec && (st = build_ctl_graph(c, p)) != EP_OK) return st;
+    DevCtl &H = *c->h_ctl;
+    ctl_upload_state(c, H);
+    EP_HIP(hipMemcpyAsync(c->d_ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
+    c->h_scal->sel = c->cur;
+    c->h_scal->halt = 0;
+    push_scalars(*c, c->stream);
+    launch_step_ctl(*c, p, 0, c->stream);
+    if ((st = check_launch()) != EP_OK) return st;
+    for (;;) {
+        for (int r = 0; r < kCtlReplays; ++r) EP_HIP(hipGraphLaunch(c->ctl_exec, c->stream));
+        EP_HIP(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
+        if ((st = sync(*c)) != EP_OK) return st;
+        if (H.done || H.error) break;
+    }
+    const int n = std::min(H.n_log, c->ctl_cap);
+    *n_iter = n;
+    if (log && log_cap > 0 && n > 0)
+        EP_HIP(hipMemcpyAsync(log, c->d_log, sizeof(ep_step_info) * std::min(n, log_cap),
+                              hipMemcpyDeviceToHost, c->stream));
+    const size_t room = kMaxStepRecords - std::min(kMaxStepRecords, c->rec_target.size());
+    const size_t nr = std::min((size_t)std::min(H.n_rec, c->ctl_cap), room);
+    std::vector<float> rec(nr * 17);
+    if (nr > 0)
+        EP_HIP(hipMemcpyAsync(rec.data(), c->d_rec, sizeof(float) * rec.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+    EP_HIP(hipMemcpyAsync(c->h_res, c->d_res, sizeof(DevResult), hipMemcpyDeviceToHost, c->stream));
+    // host-controlled launches after the run must not see the halt flag
+    c->h_scal->halt = 0;
+    push_scalars(*c, c->stream);
+    if ((st = sync(*c)) != EP_OK) return st;
+    for (size_t r = 0; r < nr; ++r) {
+        c->rec_feat.insert(c->rec_feat.end(), &rec[r * 17], &rec[r * 17] + 16);
+        c->rec_target.push_back(rec[r * 17 + 16]);
+    }
+    ctl_download_state(c, H);
+    if (H.error) return fail(EP_ENAN, "ep_run: non-finite gradient at iteration %d", c->iteration);
     return EP_OK;
 }
 
@@ -1048,13 +1215,18 @@ extern "C" ep_status ep_run(ep_ctx *c, ep_run_info *info, ep_step_info *log, int
     ep_status st;
     const int64_t ev0 = c->evals;
     if (!c->opt_ready && (st = ep_init_optimizer(c)) != EP_OK) return st;
-    int n = 0;
-    while (c->iteration < c->prm.max_iters) {
-        ep_step_info si;
-        if ((st = ep_step(c, &si)) != EP_OK) return st;
-        if (log && n < log_cap) log[n] = si;
-        ++n;
-        if (si.overflow <= c->prm.stop_overflow) break;
+    if (c->prm.use_graph && c->iteration < c->prm.max_iters) {
+        int n = 0;
+        if ((st = run_device(c, log, log_cap, &n)) != EP_OK) return st;
+    } else {
+        int n = 0;
+        while (c->iteration < c->prm.max_iters) {
+            ep_step_info si;
+            if ((st = ep_step(c, &si)) != EP_OK) return st;
+            if (log && n < log_cap) log[n] = si;
+            ++n;
+            if (si.overflow <= c->prm.stop_overflow) break;
+        }
     }
     auto t1 = std::chrono::steady_clock::now();
     if (info) {
@@ -1193,11 +1365,7 @@ extern "C" ep_status ep_set_step_model(ep_ctx *c, const float *w, int32_t n)
     }
     EP_HIP(hipMemcpy(c->d_mlp_w, w, sizeof(float) * n, hipMemcpyHostToDevice));
     c->mlp_loaded = true;
-    for (int i = 0; i < 2; ++i) {  // captured graphs must now include the proposal kernel
-        if (c->graph_exec[i]) (void)hipGraphExecDestroy(c->graph_exec[i]);
-        c->graph_exec[i] = nullptr;
-        c->graph_built[i] = false;
-    }
+    drop_graphs(*c);  // captured graphs must now include the proposal kernel
     return EP_OK;
 }
 

@@ -99,8 +99,9 @@ struct BucketTable {
     int32_t n;
 };
 
-// Scalars the kernels read each launch; staged from pinned host memory so a
-// captured graph picks up new values on every replay.
+// Scalars the kernels read each launch.  Host-controlled steps stage them from
+// pinned host memory (a captured graph picks up new values on every replay);
+// device-controlled runs have the step-control kernel write them in place.
 struct DevScalars {
     float alpha;      // step length of the current trial
     float coef;       // (a_k - 1)/a_{k+1}
@@ -108,8 +109,44 @@ struct DevScalars {
     float lambda;     // density weight
     float probe_eps;  // displacement factor of the alpha_0 probe
     int32_t use_model;  // 1: the MLP proposal kernel overwrites alpha
-    int32_t pad[2];
+    int32_t sel;        // index into the Pair arguments of a launch (= cur on device)
+    int32_t halt;       // 1: every iteration kernel returns at once (run finished)
     float feat[16];     // step-model features of the current iteration
+};
+
+// Two candidate buffers of a kernel argument; the kernel uses p[sc->sel].  A
+// launch with fixed buffers passes the same pointer twice.  The device-controlled
+// step graph passes {V[0], V[1]} for the current point and {V[1], V[0]} for the
+// trial point, so one captured graph follows the accept decisions (sel = cur)
+// made on the device.
+template <typename T>
+struct Pair {
+    T *p[2];
+};
+template <typename T>
+inline Pair<T> same(T *x)
+{
+    return Pair<T>{{x, x}};
+}
+
+// Optimiser state of a device-controlled run (ep_run with graphs): the host
+// fields of Ctx below, advanced by k_step_ctl after every trial.
+struct DevCtl {
+    double alpha, ak, ak1, coef, lambda, gamma, prev_hpwl;
+    double last_hpwl, last_tau, last_wa;
+    double h_alpha1, h_alpha2, h_prev_tau, h_prev_hpwl, h_ratio, h_gnorm, h_dv, h_dg, h_sum_ratio;
+    int64_t evals;
+    int32_t iteration, h_evals, trials, phase, t, learned_used, learned_ok;
+    int32_t learned_proposals, learned_accepted;
+    int32_t cur, done, error, n_log, n_rec;
+    float feat[16];
+};
+
+// Constant inputs of k_step_ctl.
+struct CtlParams {
+    double mu_min, mu_max, hpwl_ref, stop_overflow, gamma_base;  // gamma_base = gamma_bins * bw
+    int32_t gamma_schedule, max_iters, max_backtrack, learned, log_cap, rec_cap;
+    double bw;
 };
 
 // Reductions of one gradient evaluation (FP64), written by the finalize kernel.
@@ -129,19 +166,39 @@ struct Ctx;
 void launch_fill_maps(Ctx &c, hipStream_t s);
 // Tile-sorts the cells at pos; `counted`: the tile counts were already made by
 // a counting launch_trial at the same positions.
-void launch_sort(Ctx &c, const float2 *pos, bool counted, hipStream_t s);
+void launch_sort(Ctx &c, Pair<const float2> pos, bool counted, hipStream_t s);
 // Scatters the cells packed by the last launch_sort (same positions).
 void launch_density(Ctx &c, hipStream_t s);
 void launch_overflow(Ctx &c, hipStream_t s);
-void launch_wa(Ctx &c, const float2 *pos, hipStream_t s);
+void launch_wa(Ctx &c, Pair<const float2> pos, hipStream_t s);
 void launch_field(Ctx &c, int precision, hipStream_t s);
-void launch_cell_grad(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref,
-                      float2 *gout, float2 *gw_out, float2 *f_out, hipStream_t s);
+void launch_cell_grad(Ctx &c, Pair<const float2> pos, Pair<const float2> vref,
+                      Pair<const float2> gref, Pair<float2> gout, float2 *gw_out, float2 *f_out,
+                      hipStream_t s);
 void launch_finalize(Ctx &c, hipStream_t s);
 // Nesterov trial into v_new/u_new; `count`: also count v_new into the sort's
 // tiles (the following evaluation then sorts with counted = true).
-void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, float2 *v_new,
-                  float2 *u_new, bool count, hipStream_t s);
+void launch_trial(Ctx &c, Pair<const float2> v, Pair<const float2> u, Pair<const float2> g,
+                  Pair<float2> v_new, Pair<float2> u_new, bool count, hipStream_t s);
+// Step decision after a trial (decide = 1) or start of the first iteration
+// (decide = 0) of a device-controlled run.
+void launch_step_ctl(Ctx &c, const CtlParams &p, int decide, hipStream_t s);
+// Fixed-buffer forms of the launchers above.
+inline void launch_sort(Ctx &c, const float2 *pos, bool counted, hipStream_t s)
+{
+    launch_sort(c, same(pos), counted, s);
+}
+inline void launch_wa(Ctx &c, const float2 *pos, hipStream_t s) { launch_wa(c, same(pos), s); }
+inline void launch_cell_grad(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref,
+                             float2 *gout, float2 *gw_out, float2 *f_out, hipStream_t s)
+{
+    launch_cell_grad(c, same(pos), same(vref), same(gref), same(gout), gw_out, f_out, s);
+}
+inline void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, float2 *v_new,
+                         float2 *u_new, bool count, hipStream_t s)
+{
+    launch_trial(c, same(v), same(u), same(g), same(v_new), same(u_new), count, s);
+}
 void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s);
 void launch_mlp_propose(Ctx &c, hipStream_t s);
 void launch_wa_packet(const float *data, int net_size, int n_nets, int groups, float ginv,
@@ -206,9 +263,15 @@ struct Ctx {
     int32_t iteration = 0;
     int64_t evals = 0;
 
-    // graphs: one per parity of `cur`
+    // graphs: one per parity of `cur` (host-controlled steps)
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     bool graph_built[2] = {false, false};
+    // device-controlled runs: one trial + step decision per replay
+    hipGraphExec_t ctl_exec = nullptr;
+    DevCtl *d_ctl = nullptr, *h_ctl = nullptr;   // device / pinned host
+    ep_step_info *d_log = nullptr;               // accepted iterations of the run
+    float *d_rec = nullptr;                      // step records: 17 floats each
+    int32_t ctl_cap = 0;                         // entries of d_log and d_rec
 
     // learned step (MLP proposal)
     float *d_mlp_w = nullptr;

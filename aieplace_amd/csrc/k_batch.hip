@@ -14,6 +14,7 @@
 #include "ep_batch.h"
 #include "ep_ctx.h"
 #include "ep_device.h"
+#include "ep_sched.h"
 
 namespace ep {
 
@@ -424,14 +425,16 @@ __global__ __launch_bounds__(kBlock) void kb_reduce(BDev d)
 
 __device__ inline double gamma_dev(const BInst &I, const BCtl &C, double tau)
 {
+#pragma clang fp contract(off)
     double g = I.gamma_base;
-    if (C.gamma_schedule) g *= pow(10.0, (tau - 0.1) * 20.0 / 9.0 - 1.0);
+    if (C.gamma_schedule) g *= sched_exp10((tau - 0.1) * 20.0 / 9.0 - 1.0);
     return g;
 }
 
 // Scalars of the coming iteration's first trial (ePlace Nesterov coefficients).
 __device__ inline void begin_iteration(BState &S)
 {
+#pragma clang fp contract(off)
     S.ak1 = 0.5 * (1.0 + sqrt(4.0 * S.ak * S.ak + 1.0));
     S.coef = (S.ak - 1.0) / S.ak1;
     S.alpha_try = S.alpha;
@@ -447,6 +450,7 @@ __device__ inline void begin_iteration(BState &S)
 // trials), then lambda *= mu(dHPWL), gamma(tau), stop at overflow / max_iters.
 __global__ void kb_control(BDev d, int mode)
 {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.n_inst) return;
     BState S = d.st[i];
@@ -511,7 +515,7 @@ __global__ void kb_control(BDev d, int mode)
         S.last_tau = tau;
         S.last_wa = R.wa;
         const double dh = hp - S.prev_hpwl;
-        double mu = dh < 0 ? C.mu_max : pow(C.mu_max, 1.0 - dh / I.hpwl_ref);
+        double mu = dh < 0 ? C.mu_max : sched_pow(C.mu_max, 1.0 - dh / I.hpwl_ref);
         mu = fmin(fmax(mu, C.mu_min), C.mu_max);
         S.lambda *= mu;
         S.prev_hpwl = hp;

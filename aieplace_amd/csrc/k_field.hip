@@ -66,7 +66,7 @@ struct GemmArgs {
     const uint32_t *bmax;       // FP16X3: max |data operand| (float bits)
     uint32_t *cmax;             // FP16X3: running max |output| of this product
     const float *bbias;         // nullable: constant subtracted from the data operand
-    const int32_t *skip;        // batched: nonzero when the instance has finished
+    const int32_t *skip;        // nonzero: the product is skipped (finished run or instance)
 };
 
 __device__ inline float spectral_val(const GemmArgs &a, int row, int col, float v, float *o2)
@@ -252,6 +252,7 @@ __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
 template <bool INV, int EPI>
 __global__ __launch_bounds__(256) void k_sym_f32(GemmArgs a)
 {
+    if (a.skip && *a.skip) return;
     sym_f32_tile<INV, EPI>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
@@ -416,6 +417,7 @@ __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
 template <bool INV, int EPI, bool F16>
 __global__ __launch_bounds__(256) void k_sym_split(GemmArgs a)
 {
+    if (a.skip && *a.skip) return;
     sym_split_tile<INV, EPI, F16>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
@@ -750,6 +752,7 @@ void launch_field(Ctx &c, int precision, hipStream_t s)
     a.scale = (float)(4.0 / ((double)c.Mx * c.My) / ((double)c.bw * c.bh));
     a.wxs = (float)(M_PI / c.Mx);
     a.wys = (float)(M_PI / c.My * ((double)c.bw / c.bh));
+    a.skip = &c.d_scal->halt;  // a finished device-controlled run skips the solve
     auto setA = [&](int z, int m) { set_operand(a, z, fm->half[m], m, precision); };
     if (f16) hipLaunchKernelGGL(k_clear_max, dim3(1), dim3(64), 0, s, fm->fmax + 1, 3);
     // G1: T1t[v][i] = Cy * rho^T

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(MH) void k_mlp_propose(const float *__restrict__ w,
                                                     DevScalars *__restrict__ sc)
 {
     __shared__ float xn[MI], a1[MH], a2[MH];
-    if (!sc->use_model) return;  // uniform across the block
+    if (!sc->use_model || sc->halt) return;  // uniform across the block
     const int j = threadIdx.x;
     if (j < MI) xn[j] = (sc->feat[j] - w[j]) / w[16 + j];
     __syncthreads();

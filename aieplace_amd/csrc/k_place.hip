@@ -15,6 +15,7 @@
 
 #include "ep_ctx.h"
 #include "ep_device.h"
+#include "ep_sched.h"
 
 namespace ep {
 
@@ -25,8 +26,10 @@ using namespace dev;
 
 // Clears the fixed-point scatter accumulators (field and movable planes).
 __global__ void k_fill_maps(unsigned long long *__restrict__ acc_field,
-                            unsigned long long *__restrict__ acc_mov, int nb)
+                            unsigned long long *__restrict__ acc_mov, int nb,
+                            const DevScalars *__restrict__ sc)
 {
+    if (sc->halt) return;
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
         acc_field[b] = 0ull;
         acc_mov[b] = 0ull;
@@ -39,11 +42,14 @@ __global__ void k_fill_maps(unsigned long long *__restrict__ acc_field,
 // accumulate in LDS and flush with coalesced atomics.
 
 
-__global__ __launch_bounds__(kBlock) void k_sort_count(const float2 *__restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_sort_count(Pair<const float2> pos_pair,
                                                        const float2 *__restrict__ wh, int n_place,
                                                        GeomArgs g, int Mx, int My,
-                                                       int32_t *__restrict__ count)
+                                                       int32_t *__restrict__ count,
+                                                       const DevScalars *__restrict__ sc)
 {
+    if (sc->halt) return;
+    const float2 *__restrict__ pos = pos_pair.p[sc->sel];
     __shared__ int hist[kSortTiles];
     for (int t = threadIdx.x; t < kSortTiles; t += kBlock) hist[t] = 0;
     __syncthreads();
@@ -63,19 +69,24 @@ __global__ __launch_bounds__(kBlock) void k_sort_count(const float2 *__restrict_
 __global__ __launch_bounds__(kSortScanThreads) void k_sort_scan(int32_t *__restrict__ count,
                                                                 int32_t *__restrict__ off,
                                                                 int32_t *__restrict__ dtab,
-                                                                int level, int K)
+                                                                int level, int K,
+                                                                const DevScalars *__restrict__ sc)
 {
+    if (sc->halt) return;
     sort_scan_block(count, off, dtab, level, K);
 }
 
 // Writes every cell, packed with its size and movable flag (pack_cell), to its
 // slot of the tile-sorted order.
-__global__ __launch_bounds__(kBlock) void k_sort_scatter(const float2 *__restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(Pair<const float2> pos_pair,
                                                          const float2 *__restrict__ wh, int n_place,
                                                          int n_mov, GeomArgs g, int Mx, int My,
                                                          int32_t *__restrict__ off,
-                                                         float4 *__restrict__ sorted)
+                                                         float4 *__restrict__ sorted,
+                                                         const DevScalars *__restrict__ sc)
 {
+    if (sc->halt) return;
+    const float2 *__restrict__ pos = pos_pair.p[sc->sel];
     __shared__ int hist[kSortTiles];
     __shared__ int base_of[kSortTiles];
     for (int t = threadIdx.x; t < kSortTiles; t += kBlock) hist[t] = 0;
@@ -115,8 +126,10 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4 *__restrict__ s
                                                     int ext, int wcap, int n_place, GeomArgs g,
                                                     int Mx, int My,
                                                     unsigned long long *__restrict__ acc_field,
-                                                    unsigned long long *__restrict__ acc_mov)
+                                                    unsigned long long *__restrict__ acc_mov,
+                                                    const DevScalars *__restrict__ sc)
 {
+    if (sc->halt) return;
     const int ns = kSortTiles >> (2 * level);
     const int b = xcd_run_block(blockIdx.x, gridDim.x, kXcdRun);  // neighbouring super tiles share an L2
     if (b >= dtab[ns]) return;
@@ -139,8 +152,10 @@ __global__ __launch_bounds__(kBlock) void k_overflow(const unsigned long long *_
                                                      float rho_t, float cap, int nb,
                                                      float *__restrict__ map_field,
                                                      float *__restrict__ map_mov,
-                                                     double *__restrict__ part)
+                                                     double *__restrict__ part,
+                                                     const DevScalars *__restrict__ sc)
 {
+    if (sc->halt) return;
     double acc[4] = {0.0, 0.0, -1e300, -1e300};
     for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock) {
         const float fx = map_fix[b];
@@ -162,8 +177,10 @@ __global__ __launch_bounds__(kBlock) void k_overflow(const unsigned long long *_
 // max |map - mean| -> field_max0 (float bits).  One block.
 __global__ __launch_bounds__(kBlock) void k_map_stats(const double *__restrict__ part, int blocks,
                                                       int nb, float *__restrict__ mean_out,
-                                                      uint32_t *__restrict__ field_max0)
+                                                      uint32_t *__restrict__ field_max0,
+                                                      const DevScalars *__restrict__ sc)
 {
+    if (sc->halt) return;
     double v[3] = {0.0, -1e300, -1e300};
     for (int b = threadIdx.x; b < blocks; b += kBlock) {
         v[0] += part[4 * b + 1];
@@ -209,11 +226,13 @@ __device__ inline void wa_bucket(const Bucket &bk, int blk, const int32_t *__res
 __global__ __launch_bounds__(kBlock) void k_wa(BucketTable bt, const int32_t *__restrict__ net_start,
                                                const int32_t *__restrict__ pin_cell,
                                                const float2 *__restrict__ pin_off,
-                                               const float2 *__restrict__ pos,
+                                               Pair<const float2> pos_pair,
                                                float2 *__restrict__ pin_grad,
                                                const DevScalars *__restrict__ sc,
                                                int total_blocks, double *__restrict__ part)
 {
+    if (sc->halt) return;
+    const float2 *__restrict__ pos = pos_pair.p[sc->sel];
     const float ginv = sc->gamma_inv;
     double hp = 0.0, wa = 0.0;
     // one net tile per block (virtual blocks if the launch is capped); runs of
@@ -280,13 +299,19 @@ __global__ void k_wa_packet(const float *__restrict__ data, int net_size, int n_
 // g = (gW - lambda * F) / max(1, pins + lambda * area); F = r * sum ov * E over
 // the smoothed rectangle.  Also the FP64 partial norms for the step length.
 __global__ __launch_bounds__(kBlock) void k_cell_grad(
-    const float2 *__restrict__ pos, const float2 *__restrict__ wh, int n_place, int n_mov,
+    Pair<const float2> pos_pair, const float2 *__restrict__ wh, int n_place, int n_mov,
     GeomArgs g, int Mx, int My, const int32_t *__restrict__ cell_pin_start,
     const int32_t *__restrict__ cell_pins, const float2 *__restrict__ pin_grad,
-    const float *__restrict__ ex, const float *__restrict__ ey, const float2 *__restrict__ vref,
-    const float2 *__restrict__ gref, float2 *__restrict__ gout, float2 *__restrict__ gw_out,
+    const float *__restrict__ ex, const float *__restrict__ ey, Pair<const float2> vref_pair,
+    Pair<const float2> gref_pair, Pair<float2> gout_pair, float2 *__restrict__ gw_out,
     float2 *__restrict__ f_out, const DevScalars *__restrict__ sc, double *__restrict__ part)
 {
+    if (sc->halt) return;
+    const int sel = sc->sel;
+    const float2 *__restrict__ pos = pos_pair.p[sel];
+    const float2 *__restrict__ vref = vref_pair.p[sel];
+    const float2 *__restrict__ gref = gref_pair.p[sel];
+    float2 *__restrict__ gout = gout_pair.p[sel];
     const float lambda = sc->lambda;
     // sums: dv2, dg2, |gw|, |f|, g2, nonfinite ; max: |g|
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -334,6 +359,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     int cell_off, int cell_blocks, double mov_area, const DevScalars *__restrict__ sc,
     DevResult *__restrict__ res)
 {
+    if (sc->halt) return;
     double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < wa_blocks; b += kFinalizeThreads) {
         v[0] += part[wa_off + 2 * b];
@@ -369,16 +395,23 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
 // With `count`, the new positions are also counted into the sort's coarse
 // tiles (the first pass of launch_sort, which then skips it).  Blocks take
 // kBlock * kSortCellsPerThread cells, as k_sort_count.
-__global__ __launch_bounds__(kBlock) void k_trial(const float2 *__restrict__ v,
-                                                  const float2 *__restrict__ u,
-                                                  const float2 *__restrict__ gr,
+__global__ __launch_bounds__(kBlock) void k_trial(Pair<const float2> v_pair,
+                                                  Pair<const float2> u_pair,
+                                                  Pair<const float2> g_pair,
                                                   const float2 *__restrict__ wh, int n_place,
                                                   GeomArgs g, int Mx, int My,
                                                   const DevScalars *__restrict__ sc,
-                                                  float2 *__restrict__ v_new,
-                                                  float2 *__restrict__ u_new,
+                                                  Pair<float2> v_new_pair,
+                                                  Pair<float2> u_new_pair,
                                                   int32_t *__restrict__ count)
 {
+    if (sc->halt) return;
+    const int sel = sc->sel;
+    const float2 *__restrict__ v = v_pair.p[sel];
+    const float2 *__restrict__ u = u_pair.p[sel];
+    const float2 *__restrict__ gr = g_pair.p[sel];
+    float2 *__restrict__ v_new = v_new_pair.p[sel];
+    float2 *__restrict__ u_new = u_new_pair.p[sel];
     __shared__ int hist[kSortTiles];
     if (count) {
         for (int t = threadIdx.x; t < kSortTiles; t += kBlock) hist[t] = 0;
@@ -436,6 +469,163 @@ __global__ void k_cell_bins(const float2 *__restrict__ pos, const float2 *__rest
     }
 }
 
+// ------------------------------------------------------------------ step control
+//
+// The decision of ep_step (ep_api.cpp) after every trial of a device-controlled
+// run, on one thread: accept when alpha_hat >= 0.95 alpha_used (the model's
+// proposal first, then the Lipschitz search, at most max_backtrack trials),
+// then lambda *= mu(dHPWL), gamma(tau), stop at the overflow target or
+// max_iters.  Same FP64 operations in the same order as the host, without
+// multiply-add contraction (which the host compiler does not do either), so
+// both controls take the same decisions.
+
+__device__ inline float lg10_clamped(double v) { return (float)sched_log10(fmax(v, 1e-30)); }
+
+// Step-model features of the coming iteration (order: include/eplace_mlp.h).
+__device__ inline void ctl_features(const DevCtl &S, double bw, float *f)
+{
+#pragma clang fp contract(off)
+    f[0] = lg10_clamped(S.alpha);
+    f[1] = lg10_clamped(S.h_alpha1);
+    f[2] = lg10_clamped(S.h_alpha2);
+    f[3] = (float)S.last_tau;
+    f[4] = (float)(S.last_tau - S.h_prev_tau);
+    f[5] = lg10_clamped(S.lambda);
+    f[6] = lg10_clamped(S.gamma / bw);
+    f[7] = (float)((S.last_hpwl - S.h_prev_hpwl) / fmax(S.last_hpwl, 1e-30));
+    f[8] = lg10_clamped(S.h_gnorm);
+    f[9] = lg10_clamped(S.h_dv);
+    f[10] = lg10_clamped(S.h_dg);
+    f[11] = (float)S.coef;
+    f[12] = (float)S.h_evals;
+    f[13] = lg10_clamped(S.h_sum_ratio);
+    f[14] = lg10_clamped(S.h_ratio);
+    f[15] = (float)(S.iteration / 1000.0);
+}
+
+// Nesterov coefficient, features and the scalars of the iteration's first trial.
+__device__ inline void ctl_begin(DevCtl &S, DevScalars *sc, const CtlParams &p)
+{
+#pragma clang fp contract(off)
+    S.ak1 = 0.5 * (1.0 + sqrt(4.0 * S.ak * S.ak + 1.0));
+    S.coef = (S.ak - 1.0) / S.ak1;
+    ctl_features(S, p.bw, S.feat);
+    for (int k = 0; k < 16; ++k) sc->feat[k] = S.feat[k];
+    S.trials = 0;
+    S.t = 0;
+    S.learned_ok = 0;
+    S.learned_used = p.learned ? 1 : 0;
+    S.phase = p.learned ? 1 : 0;
+    if (p.learned) S.learned_proposals++;
+    sc->alpha = (float)S.alpha;
+    sc->coef = (float)S.coef;
+    sc->gamma_inv = (float)(1.0 / S.gamma);
+    sc->lambda = (float)S.lambda;
+    sc->use_model = p.learned ? 1 : 0;
+}
+
+__global__ void k_step_ctl(DevCtl *__restrict__ ctl, const DevResult *__restrict__ res,
+                           DevScalars *__restrict__ sc, CtlParams p, int decide,
+                           ep_step_info *__restrict__ log, float *__restrict__ rec)
+{
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    DevCtl S = *ctl;
+    if (S.done) return;
+    if (!decide) {
+        ctl_begin(S, sc, p);
+        *ctl = S;
+        return;
+    }
+    const DevResult R = *res;
+    S.evals++;
+    S.trials++;
+    if (R.nonfinite) {
+        S.error = 1;
+        S.done = 1;
+        sc->halt = 1;
+        *ctl = S;
+        return;
+    }
+    const double used = R.alpha_used;
+    const double nn = sqrt(R.dv2), dd = sqrt(R.dg2);
+    const double ahat = dd > 0 ? nn / dd : used;
+    bool accept = false;
+    if (S.phase == 1) {
+        if (isfinite(ahat) && ahat >= 0.95 * used) {
+            accept = true;
+            S.learned_ok = 1;
+            S.learned_accepted++;
+        } else {  // fall back to the search from the Lipschitz step
+            S.phase = 0;
+            S.t = 0;
+            sc->use_model = 0;
+            sc->alpha = (float)S.alpha;
+        }
+    } else if (ahat >= 0.95 * used || S.t == p.max_backtrack - 1) {
+        accept = true;
+    } else {
+        S.t++;
+        sc->alpha = (float)ahat;
+    }
+    if (accept) {
+        if (S.n_rec < p.rec_cap) {
+            float *r = rec + (size_t)S.n_rec * 17;
+            for (int k = 0; k < 16; ++k) r[k] = S.feat[k];
+            r[16] = (float)sched_log10(fmax(used, 1e-30));
+            S.n_rec++;
+        }
+        S.cur ^= 1;
+        sc->sel = S.cur;
+        S.alpha = ahat;
+        S.ak = S.ak1;
+        const double hp = R.hpwl, tau = R.tau;
+        S.h_alpha2 = S.h_alpha1;
+        S.h_alpha1 = used;
+        S.h_prev_tau = S.last_tau;
+        S.h_prev_hpwl = S.last_hpwl;
+        S.h_ratio = ahat / fmax(used, 1e-30);
+        S.h_gnorm = sqrt(R.gnorm2);
+        S.h_dv = nn;
+        S.h_dg = dd;
+        S.h_sum_ratio = R.sum_f > 0 ? R.sum_gw / R.sum_f : 1.0;
+        S.h_evals = S.trials;
+        S.last_hpwl = hp;
+        S.last_tau = tau;
+        S.last_wa = R.wa;
+        if (S.n_log < p.log_cap) {
+            ep_step_info &e = log[S.n_log];
+            e.iteration = S.iteration;
+            e.evals = S.trials;
+            e.learned_used = S.learned_used;
+            e.learned_accepted = S.learned_ok;
+            e.hpwl = hp;
+            e.overflow = tau;
+            e.lambda = S.lambda;
+            e.gamma = S.gamma;
+            e.alpha = S.alpha;
+            e.wa = R.wa;
+        }
+        S.n_log++;
+        const double dh = hp - S.prev_hpwl;
+        double mu = dh < 0 ? p.mu_max : sched_pow(p.mu_max, 1.0 - dh / p.hpwl_ref);
+        mu = fmin(fmax(mu, p.mu_min), p.mu_max);
+        S.lambda *= mu;
+        S.prev_hpwl = hp;
+        double gam = p.gamma_base;
+        if (p.gamma_schedule) gam *= sched_exp10((tau - 0.1) * 20.0 / 9.0 - 1.0);
+        S.gamma = gam;
+        S.iteration++;
+        if (tau <= p.stop_overflow || S.iteration >= p.max_iters) {
+            S.done = 1;
+            sc->halt = 1;
+        } else {
+            ctl_begin(S, sc, p);
+        }
+    }
+    *ctl = S;
+}
+
 GeomArgs geom_args(const Ctx &c)
 {
     GeomArgs g;
@@ -460,19 +650,19 @@ void launch_fill_maps(Ctx &c, hipStream_t s)
 {
     const int nb = c.Mx * c.My;
     hipLaunchKernelGGL(k_fill_maps, dim3(grid_for(nb, kBlock * 4, 4096)), dim3(kBlock), 0, s,
-                       c.acc_field, c.acc_mov, nb);
+                       c.acc_field, c.acc_mov, nb, c.d_scal);
 }
 
-void launch_sort(Ctx &c, const float2 *pos, bool counted, hipStream_t s)
+void launch_sort(Ctx &c, Pair<const float2> pos, bool counted, hipStream_t s)
 {
     const int blocks = grid_for(c.n_place, kBlock * kSortCellsPerThread, INT_MAX);
     if (!counted)
         hipLaunchKernelGGL(k_sort_count, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
-                           geom_args(c), c.Mx, c.My, c.sort_count);
+                           geom_args(c), c.Mx, c.My, c.sort_count, c.d_scal);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortScanThreads), 0, s, c.sort_count, c.sort_off,
-                       c.dens_tab, c.dens.level, kBlock * c.dens.cpt);
+                       c.dens_tab, c.dens.level, kBlock * c.dens.cpt, c.d_scal);
     hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
-                       c.n_mov, geom_args(c), c.Mx, c.My, c.sort_off, c.sorted);
+                       c.n_mov, geom_args(c), c.Mx, c.My, c.sort_off, c.sorted, c.d_scal);
 }
 
 void launch_density(Ctx &c, hipStream_t s)
@@ -483,7 +673,7 @@ void launch_density(Ctx &c, hipStream_t s)
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, s, c.sorted, c.sort_off, c.dens_tab,
                            dp.level, dp.ext, dp.wcap, c.n_place, geom_args(c), c.Mx, c.My,
-                           c.acc_field, c.acc_mov);
+                           c.acc_field, c.acc_mov, c.d_scal);
     };
     if (dp.cpt == 2) go(k_density<2>);
     else if (dp.cpt == 8) go(k_density<8>);
@@ -497,12 +687,12 @@ void launch_overflow(Ctx &c, hipStream_t s)
     const double unit = (double)c.bw * (double)c.bh / kDensFixUnits;
     hipLaunchKernelGGL(k_overflow, dim3(c.ovf_blocks), dim3(kBlock), 0, s, c.acc_field, c.acc_mov,
                        c.map_fix, unit, c.prm.target_density, cap, nb, c.map_field, c.map_mov,
-                       c.part + c.part_ovf_off);
+                       c.part + c.part_ovf_off, c.d_scal);
     hipLaunchKernelGGL(k_map_stats, dim3(1), dim3(kBlock), 0, s, c.part + c.part_ovf_off,
-                       c.ovf_blocks, nb, c.map_mean, c.field_max);
+                       c.ovf_blocks, nb, c.map_mean, c.field_max, c.d_scal);
 }
 
-void launch_wa(Ctx &c, const float2 *pos, hipStream_t s)
+void launch_wa(Ctx &c, Pair<const float2> pos, hipStream_t s)
 {
     if (c.wa_blocks == 0) return;
     hipLaunchKernelGGL(k_wa, dim3(c.wa_part_blocks), dim3(kBlock), 0, s, c.buckets, c.net_start,
@@ -510,8 +700,9 @@ void launch_wa(Ctx &c, const float2 *pos, hipStream_t s)
                        c.part + c.part_wa_off);
 }
 
-void launch_cell_grad(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref,
-                      float2 *gout, float2 *gw_out, float2 *f_out, hipStream_t s)
+void launch_cell_grad(Ctx &c, Pair<const float2> pos, Pair<const float2> vref,
+                      Pair<const float2> gref, Pair<float2> gout, float2 *gw_out, float2 *f_out,
+                      hipStream_t s)
 {
     hipLaunchKernelGGL(k_cell_grad, dim3(c.cell_blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
                        c.n_mov, geom_args(c), c.Mx, c.My, c.cell_pin_start, c.cell_pins,
@@ -526,13 +717,19 @@ void launch_finalize(Ctx &c, hipStream_t s)
                        c.cell_blocks, c.mov_area, c.d_scal, c.d_res);
 }
 
-void launch_trial(Ctx &c, const float2 *v, const float2 *u, const float2 *g, float2 *v_new,
-                  float2 *u_new, bool count, hipStream_t s)
+void launch_trial(Ctx &c, Pair<const float2> v, Pair<const float2> u, Pair<const float2> g,
+                  Pair<float2> v_new, Pair<float2> u_new, bool count, hipStream_t s)
 {
     const int blocks = grid_for(c.n_place, kBlock * kSortCellsPerThread, INT_MAX);
     hipLaunchKernelGGL(k_trial, dim3(blocks), dim3(kBlock), 0, s, v, u, g, c.wh, c.n_place,
                        geom_args(c), c.Mx, c.My, c.d_scal, v_new, u_new,
                        count ? c.sort_count : nullptr);
+}
+
+void launch_step_ctl(Ctx &c, const CtlParams &p, int decide, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_ctl, dim3(1), dim3(64), 0, s, c.d_ctl, c.d_res, c.d_scal, p, decide,
+                       c.d_log, c.d_rec);
 }
 
 void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s)

@@ -265,3 +265,38 @@ def test_placement_is_deterministic(precision):
     assert r1["hpwl"] == r2["hpwl"] and r1["overflow"] == r2["overflow"]
     assert [l["hpwl"] for l in r1["log"]] == [l["hpwl"] for l in r2["log"]]
     assert np.array_equal(x1, x2) and np.array_equal(y1, y2)
+
+
+@pytest.mark.parametrize("precision,learned_step", [(api.EP_PREC_FP32, 0), (api.EP_PREC_BF16X3, 0),
+                                                    (api.EP_PREC_BF16X3, 1)])
+def test_device_control_matches_host_control(precision, learned_step):
+    """ep_run with graphs decides every trial on the device (the step-control
+    kernel replays ep_step's FP64 decisions); without graphs ep_run loops over
+    the host's ep_step.  Both take the same decisions, so the runs agree
+    exactly: iteration and evaluation counts, every logged iteration, the
+    step records and the final positions."""
+    from aieplace_amd import learned
+    say(f"case device vs host step control precision={precision} learned={learned_step}")
+    nl, p = make("stdcell", 20_000, 128, precision=precision, learned_step=learned_step)
+    out = {}
+    for g in (0, 1):
+        p.use_graph = g
+        with A.Placer(nl, p) as pl:
+            if learned_step:
+                pl.set_step_model(learned.load())
+            r = pl.run(log_cap=2000)
+            x, y = pl.get_positions()
+            f, t = pl.take_step_records()
+        out[g] = (r, x, y, f, t)
+    (r0, x0, y0, f0, t0), (r1, x1, y1, f1, t1) = out[0], out[1]
+    say(f"  host: it {r0['iterations']} ev {r0['evals']} hpwl {r0['hpwl']!r} {r0['seconds']:.3f}s; "
+        f"device: it {r1['iterations']} ev {r1['evals']} hpwl {r1['hpwl']!r} {r1['seconds']:.3f}s")
+    assert r0["iterations"] == r1["iterations"] and r0["evals"] == r1["evals"]
+    assert r0["learned_proposals"] == r1["learned_proposals"]
+    assert r0["learned_accepted"] == r1["learned_accepted"]
+    assert r0["hpwl"] == r1["hpwl"] and r0["overflow"] == r1["overflow"]
+    assert len(r0["log"]) == len(r1["log"]) == r0["iterations"]
+    for a, b in zip(r0["log"], r1["log"]):
+        assert a == b
+    assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
+    assert np.array_equal(f0, f1) and np.array_equal(t0, t1)
