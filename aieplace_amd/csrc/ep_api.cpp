@@ -242,7 +242,8 @@ static void destroy_ctx(ep_ctx *c)
                     c->cell_pins, c->V[0], c->V[1], c->U[0], c->U[1], c->G[0], c->G[1],
                     c->map_field, c->map_mov, c->map_fix, c->acc_field, c->acc_mov, c->ex, c->ey, c->tA, c->tB, c->part,
                     c->sorted, c->sort_count, c->sort_off, c->dens_tab,
-                    c->d_scal, c->d_res, c->d_mlp_w, c->d_ctl, c->d_log, c->d_rec};
+                    c->d_scal, c->d_res, c->d_mlp_w, c->d_ctl, c->d_log, c->d_rec, c->fin,
+                    c->fin_ticket};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     field_free(*c);
@@ -564,6 +565,8 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     EP_TRY(dalloc(&c->tA, 2 * nb));
     EP_TRY(dalloc(&c->tB, 2 * nb));
     EP_TRY(dalloc(&c->part, c->part_total));
+    EP_TRY(dalloc(&c->fin, 32 * 10));
+    EP_TRY(dalloc(&c->fin_ticket, 1));
     EP_TRY(dalloc(&c->d_scal, 1));
     EP_TRY(dalloc(&c->d_res, 1));
     if (hipHostMalloc((void **)&c->h_scal, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess ||
@@ -593,6 +596,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
         hipMemset(c->ex, 0, sizeof(float) * nb) != hipSuccess ||
         hipMemset(c->ey, 0, sizeof(float) * nb) != hipSuccess ||
         hipMemset(c->part, 0, sizeof(double) * c->part_total) != hipSuccess ||
+        hipMemset(c->fin_ticket, 0, sizeof(unsigned)) != hipSuccess ||
         hipMemset(c->sort_count, 0, sizeof(int32_t) * kSortTiles) != hipSuccess) {
         destroy_ctx(c);
         return fail(EP_EDEVICE, "ep_create: hipMemset failed");
@@ -1055,14 +1059,15 @@ extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
 
 // ---------------------------------------------------------------- device-controlled run
 //
-// ep_run with graphs: one captured graph = model proposal, trial, evaluation
-// and k_step_ctl (the ep_step decision on the device).  The kernels take the
-// current and trial buffers from Pair arguments indexed by the device's `cur`,
-// so the same graph serves every trial; the host replays it kCtlReplays times
-// per synchronisation and reads the control state back.  After the stop the
-// remaining replays of a batch return at once (halt flag).
+// ep_run with graphs: one captured graph = kCtlTrialsPerGraph x (model
+// proposal, trial, evaluation, finalize + the ep_step decision on the device).
+// The kernels take the current and trial buffers from Pair arguments indexed
+// by the device's `cur`, so the same graph serves every trial; the host
+// replays it kCtlReplays times per synchronisation and reads the control state
+// back.  After the stop the remaining trials return at once (halt flag).
 
-constexpr int kCtlReplays = 6;
+constexpr int kCtlTrialsPerGraph = 4;
+constexpr int kCtlReplays = 2;
 
 static CtlParams ctl_params(const ep_ctx *c)
 {
@@ -1112,17 +1117,18 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
     const Pair<const float2> vt{{x.V[1], x.V[0]}};
     hipGraph_t g = nullptr;
     EP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    launch_mlp_propose(x, s);
-    launch_trial(x, vc, uc, gc, vn, un, true, s);
-    launch_fill_maps(x, s);
-    launch_sort(x, vt, true, s);
-    launch_density(x, s);
-    launch_overflow(x, s);
-    launch_wa(x, vt, s);
-    launch_field(x, x.prm.precision, s);
-    launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s);
-    launch_finalize(x, s);
-    launch_step_ctl(x, p, 1, s);
+    for (int t = 0; t < kCtlTrialsPerGraph; ++t) {
+        launch_mlp_propose(x, s);
+        launch_trial(x, vc, uc, gc, vn, un, true, s);
+        launch_fill_maps(x, s);
+        launch_sort(x, vt, true, s);
+        launch_density(x, s);
+        launch_overflow(x, s);
+        launch_wa(x, vt, s);
+        launch_field(x, x.prm.precision, s);
+        launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s);
+        launch_finalize_decide(x, p, s);
+    }
     EP_HIP(hipStreamEndCapture(s, &g));
     hipError_t e = hipGraphInstantiate(&c->ctl_exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1175,7 +1181,7 @@ static ep_status run_device(ep_ctx *c, ep_step_info *log, int32_t log_cap, int *
     c->h_scal->sel = c->cur;
     c->h_scal->halt = 0;
     push_scalars(*c, c->stream);
-    launch_step_ctl(*c, p, 0, c->stream);
+    launch_step_begin(*c, p, c->stream);
     if ((st = check_launch()) != EP_OK) return st;
     for (;;) {
         for (int r = 0; r < kCtlReplays; ++r) EP_HIP(hipGraphLaunch(c->ctl_exec, c->stream));

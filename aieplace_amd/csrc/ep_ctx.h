@@ -180,9 +180,10 @@ void launch_finalize(Ctx &c, hipStream_t s);
 // tiles (the following evaluation then sorts with counted = true).
 void launch_trial(Ctx &c, Pair<const float2> v, Pair<const float2> u, Pair<const float2> g,
                   Pair<float2> v_new, Pair<float2> u_new, bool count, hipStream_t s);
-// Step decision after a trial (decide = 1) or start of the first iteration
-// (decide = 0) of a device-controlled run.
-void launch_step_ctl(Ctx &c, const CtlParams &p, int decide, hipStream_t s);
+// Device-controlled runs: start of the first iteration, and the finalize
+// reduction followed at once by the step decision on its results.
+void launch_step_begin(Ctx &c, const CtlParams &p, hipStream_t s);
+void launch_finalize_decide(Ctx &c, const CtlParams &p, hipStream_t s);
 // Fixed-buffer forms of the launchers above.
 inline void launch_sort(Ctx &c, const float2 *pos, bool counted, hipStream_t s)
 {
@@ -251,6 +252,8 @@ struct Ctx {
 
     // reductions
     double *part = nullptr;   // partial sums, see part_* offsets
+    double *fin = nullptr;    // second-level sums of the finalize blocks (32 x 10)
+    unsigned *fin_ticket = nullptr;  // finalize blocks done (0 between launches)
     int32_t part_wa_off = 0, part_ovf_off = 0, part_cell_off = 0, part_total = 0;
     int32_t wa_part_blocks = 0, ovf_blocks = 0, cell_blocks = 0;
     DevScalars *d_scal = nullptr, *h_scal = nullptr;  // device / pinned host

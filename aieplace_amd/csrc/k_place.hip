@@ -25,14 +25,18 @@ using namespace dev;
 
 
 // Clears the fixed-point scatter accumulators (field and movable planes).
+// 16-byte stores (nb is even: bins are multiples of 16).
 __global__ void k_fill_maps(unsigned long long *__restrict__ acc_field,
                             unsigned long long *__restrict__ acc_mov, int nb,
                             const DevScalars *__restrict__ sc)
 {
     if (sc->halt) return;
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
-        acc_field[b] = 0ull;
-        acc_mov[b] = 0ull;
+    ulonglong2 *f = reinterpret_cast<ulonglong2 *>(acc_field);
+    ulonglong2 *m = reinterpret_cast<ulonglong2 *>(acc_mov);
+    const ulonglong2 z = make_ulonglong2(0ull, 0ull);
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < (nb >> 1); b += gridDim.x * blockDim.x) {
+        f[b] = z;
+        m[b] = z;
     }
 }
 
@@ -351,42 +355,90 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
     block_reduce<7, 1>(acc, part + 7 * blockIdx.x);
 }
 
-// Deterministic final reduction of all partial sums (fixed order).
-constexpr int kFinalizeThreads = 1024;
+// Deterministic final reduction of all partial sums: block b of
+// kFinalizeBlocks sums slice b of every partial array in a fixed order and
+// writes its ten values to fin[b]; the last block to finish (atomic ticket)
+// sums fin[0..B) in block order, so the result does not depend on timing.
+constexpr int kFinalizeThreads = 256;
+constexpr int kFinalizeBlocks = 32;
+
+struct StepCtlArgs {
+    DevCtl *ctl;            // null: no step decision (host-controlled launches)
+    ep_step_info *log;
+    float *rec;
+    CtlParams p;
+};
+
+__device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
+                                   DevScalars *__restrict__ sc, const CtlParams &p,
+                                   ep_step_info *__restrict__ log, float *__restrict__ rec);
+
+__device__ inline double load_agent(const double *p)
+{
+    return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
 
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     const double *__restrict__ part, int wa_off, int wa_blocks, int ovf_off, int ovf_blocks,
-    int cell_off, int cell_blocks, double mov_area, const DevScalars *__restrict__ sc,
-    DevResult *__restrict__ res)
+    int cell_off, int cell_blocks, double mov_area, DevScalars *__restrict__ sc,
+    DevResult *__restrict__ res, double *__restrict__ fin, unsigned *__restrict__ ticket,
+    StepCtlArgs ctl)
 {
     if (sc->halt) return;
+    const int B = gridDim.x, b = blockIdx.x;
+    auto slice = [&](int n, int &lo, int &hi) {
+        lo = (int)((int64_t)n * b / B);
+        hi = (int)((int64_t)n * (b + 1) / B);
+    };
     double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < wa_blocks; b += kFinalizeThreads) {
-        v[0] += part[wa_off + 2 * b];
-        v[1] += part[wa_off + 2 * b + 1];
+    int lo, hi;
+    slice(wa_blocks, lo, hi);
+    for (int i = lo + threadIdx.x; i < hi; i += kFinalizeThreads) {
+        v[0] += part[wa_off + 2 * i];
+        v[1] += part[wa_off + 2 * i + 1];
     }
-    for (int b = threadIdx.x; b < ovf_blocks; b += kFinalizeThreads) v[2] += part[ovf_off + 4 * b];
-    for (int b = threadIdx.x; b < cell_blocks; b += kFinalizeThreads) {
-        const double *q = part + cell_off + 7 * b;
+    slice(ovf_blocks, lo, hi);
+    for (int i = lo + threadIdx.x; i < hi; i += kFinalizeThreads) v[2] += part[ovf_off + 4 * i];
+    slice(cell_blocks, lo, hi);
+    for (int i = lo + threadIdx.x; i < hi; i += kFinalizeThreads) {
+        const double *q = part + cell_off + 7 * i;
         v[3] += q[0]; v[4] += q[1]; v[5] += q[2]; v[6] += q[3]; v[7] += q[4]; v[8] += q[5];
         v[9] = fmax(v[9], q[6]);
     }
     __shared__ double out[10];
+    __shared__ int last;
     block_reduce<10, 1, kFinalizeThreads>(v, out);
     if (threadIdx.x == 0) {
-        res->hpwl = out[0];
-        res->wa = out[1];
-        res->ovf_num = out[2];
-        res->tau = mov_area > 0 ? out[2] / mov_area : 0.0;
-        res->dv2 = out[3];
-        res->dg2 = out[4];
-        res->sum_gw = out[5];
-        res->sum_f = out[6];
-        res->gnorm2 = out[7];
-        res->nonfinite = (int64_t)out[8];
-        res->gmax = out[9];
-        res->alpha_used = sc->alpha;
+        for (int k = 0; k < 10; ++k) fin[b * 10 + k] = out[k];
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == (unsigned)(B - 1);
     }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    double t[10];
+    for (int k = 0; k < 10; ++k) t[k] = load_agent(&fin[k]);
+    for (int j = 1; j < B; ++j) {
+        for (int k = 0; k < 9; ++k) t[k] += load_agent(&fin[j * 10 + k]);
+        t[9] = fmax(t[9], load_agent(&fin[j * 10 + 9]));
+    }
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // next launch
+    DevResult R;
+    R.hpwl = t[0];
+    R.wa = t[1];
+    R.ovf_num = t[2];
+    R.tau = mov_area > 0 ? t[2] / mov_area : 0.0;
+    R.dv2 = t[3];
+    R.dg2 = t[4];
+    R.sum_gw = t[5];
+    R.sum_f = t[6];
+    R.gnorm2 = t[7];
+    R.nonfinite = (int64_t)t[8];
+    R.gmax = t[9];
+    R.alpha_used = sc->alpha;
+    *res = R;
+    if (ctl.ctl) step_decide(ctl.ctl, R, sc, ctl.p, ctl.log, ctl.rec);  // device-controlled run
 }
 
 // ------------------------------------------------------------------ Nesterov
@@ -524,20 +576,14 @@ __device__ inline void ctl_begin(DevCtl &S, DevScalars *sc, const CtlParams &p)
     sc->use_model = p.learned ? 1 : 0;
 }
 
-__global__ void k_step_ctl(DevCtl *__restrict__ ctl, const DevResult *__restrict__ res,
-                           DevScalars *__restrict__ sc, CtlParams p, int decide,
-                           ep_step_info *__restrict__ log, float *__restrict__ rec)
+// The decision after a trial whose reductions are R (one thread).
+__device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
+                                   DevScalars *__restrict__ sc, const CtlParams &p,
+                                   ep_step_info *__restrict__ log, float *__restrict__ rec)
 {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0) return;
     DevCtl S = *ctl;
     if (S.done) return;
-    if (!decide) {
-        ctl_begin(S, sc, p);
-        *ctl = S;
-        return;
-    }
-    const DevResult R = *res;
     S.evals++;
     S.trials++;
     if (R.nonfinite) {
@@ -626,6 +672,16 @@ __global__ void k_step_ctl(DevCtl *__restrict__ ctl, const DevResult *__restrict
     *ctl = S;
 }
 
+// Start of the first iteration of a device-controlled run.
+__global__ void k_step_begin(DevCtl *__restrict__ ctl, DevScalars *__restrict__ sc, CtlParams p)
+{
+    if (threadIdx.x != 0) return;
+    DevCtl S = *ctl;
+    if (S.done) return;
+    ctl_begin(S, sc, p);
+    *ctl = S;
+}
+
 GeomArgs geom_args(const Ctx &c)
 {
     GeomArgs g;
@@ -649,7 +705,7 @@ int grid_for(int64_t n, int per_block, int cap)
 void launch_fill_maps(Ctx &c, hipStream_t s)
 {
     const int nb = c.Mx * c.My;
-    hipLaunchKernelGGL(k_fill_maps, dim3(grid_for(nb, kBlock * 4, 4096)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_fill_maps, dim3(grid_for(nb / 2, kBlock * 2, 4096)), dim3(kBlock), 0, s,
                        c.acc_field, c.acc_mov, nb, c.d_scal);
 }
 
@@ -712,9 +768,18 @@ void launch_cell_grad(Ctx &c, Pair<const float2> pos, Pair<const float2> vref,
 
 void launch_finalize(Ctx &c, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, s, c.part, c.part_wa_off,
-                       c.wa_part_blocks, c.part_ovf_off, c.ovf_blocks, c.part_cell_off,
-                       c.cell_blocks, c.mov_area, c.d_scal, c.d_res);
+    StepCtlArgs none{};
+    hipLaunchKernelGGL(k_finalize, dim3(kFinalizeBlocks), dim3(kFinalizeThreads), 0, s, c.part,
+                       c.part_wa_off, c.wa_part_blocks, c.part_ovf_off, c.ovf_blocks, c.part_cell_off,
+                       c.cell_blocks, c.mov_area, c.d_scal, c.d_res, c.fin, c.fin_ticket, none);
+}
+
+void launch_finalize_decide(Ctx &c, const CtlParams &p, hipStream_t s)
+{
+    const StepCtlArgs a{c.d_ctl, c.d_log, c.d_rec, p};
+    hipLaunchKernelGGL(k_finalize, dim3(kFinalizeBlocks), dim3(kFinalizeThreads), 0, s, c.part,
+                       c.part_wa_off, c.wa_part_blocks, c.part_ovf_off, c.ovf_blocks, c.part_cell_off,
+                       c.cell_blocks, c.mov_area, c.d_scal, c.d_res, c.fin, c.fin_ticket, a);
 }
 
 void launch_trial(Ctx &c, Pair<const float2> v, Pair<const float2> u, Pair<const float2> g,
@@ -726,10 +791,9 @@ void launch_trial(Ctx &c, Pair<const float2> v, Pair<const float2> u, Pair<const
                        count ? c.sort_count : nullptr);
 }
 
-void launch_step_ctl(Ctx &c, const CtlParams &p, int decide, hipStream_t s)
+void launch_step_begin(Ctx &c, const CtlParams &p, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_ctl, dim3(1), dim3(64), 0, s, c.d_ctl, c.d_res, c.d_scal, p, decide,
-                       c.d_log, c.d_rec);
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, c.d_ctl, c.d_scal, p);
 }
 
 void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s)
