@@ -74,7 +74,8 @@ struct ep_batch {
     float2 *wh = nullptr, *V[2] = {}, *U[2] = {}, *G[2] = {};
     int32_t *net_start = nullptr, *pin_cell = nullptr, *cps = nullptr, *cpins = nullptr;
     float2 *pin_off = nullptr, *pin_grad = nullptr;
-    float *map_field = nullptr, *map_mov = nullptr, *map_fix = nullptr, *ex = nullptr, *ey = nullptr;
+    float *map_field = nullptr, *map_mov = nullptr, *map_fix = nullptr;
+    float2 *exy = nullptr;
     float *tA = nullptr, *tB = nullptr, *map_mean = nullptr;
     unsigned long long *acc_field = nullptr, *acc_mov = nullptr;
     uint32_t *map_max = nullptr;
@@ -124,8 +125,7 @@ struct ep_batch {
         d.map_fix = map_fix;
         d.acc_field = acc_field;
         d.acc_mov = acc_mov;
-        d.ex = ex;
-        d.ey = ey;
+        d.exy = exy;
         d.map_mean = map_mean;
         d.map_max = map_max;
         d.sorted = sorted;
@@ -165,7 +165,7 @@ static void destroy_batch(ep_batch *b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     void *ptrs[] = {b->d_inst, b->d_st, b->d_res, b->wh, b->V[0], b->V[1], b->U[0], b->U[1],
                     b->G[0], b->G[1], b->net_start, b->pin_cell, b->cps, b->cpins, b->pin_off,
-                    b->pin_grad, b->map_field, b->map_mov, b->map_fix, b->acc_field, b->acc_mov, b->ex, b->ey, b->tA, b->tB,
+                    b->pin_grad, b->map_field, b->map_mov, b->map_fix, b->acc_field, b->acc_mov, b->exy, b->tA, b->tB,
                     b->map_mean, b->map_max, b->sorted, b->sort_count, b->sort_off, b->dens_tab, b->pre_cell,
                     b->pre_sort, b->pre_bin, b->pre_wa, b->pre_dens[0], b->pre_dens[1],
                     b->pre_dens[2], b->buckets, b->part_wa, b->part_ovf, b->part_cell};
@@ -401,8 +401,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     EPB_TRY(dalloc(&b->acc_field, nb));
     EPB_TRY(dalloc(&b->acc_mov, nb));
     EPB_TRY(dalloc(&b->map_fix, nb));
-    EPB_TRY(dalloc(&b->ex, nb));
-    EPB_TRY(dalloc(&b->ey, nb));
+    EPB_TRY(dalloc(&b->exy, nb));
     EPB_TRY(dalloc(&b->tA, 2 * nb));
     EPB_TRY(dalloc(&b->tB, 2 * nb));
     EPB_TRY(dalloc(&b->map_mean, n));
@@ -439,8 +438,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     for (int v = 0; v < 3; ++v) EPB_TRY(upload(b->pre_dens[v], pre_dens[v]));
     EPB_TRY(upload(b->buckets, bks));
     if (hipMemset(b->pin_grad, 0, sizeof(float2) * std::max<size_t>(1, pin_cell.size())) != hipSuccess ||
-        hipMemset(b->ex, 0, sizeof(float) * nb) != hipSuccess ||
-        hipMemset(b->ey, 0, sizeof(float) * nb) != hipSuccess ||
+        hipMemset(b->exy, 0, sizeof(float2) * nb) != hipSuccess ||
         hipMemset(b->sort_count, 0, sizeof(int32_t) * n * kSortTiles) != hipSuccess ||
         hipMemset(b->part_wa, 0, sizeof(double) * 2 * std::max(1, b->wa_blocks)) != hipSuccess ||
         hipMemset(b->part_ovf, 0, sizeof(double) * 4 * b->bin_blocks) != hipSuccess ||
@@ -458,8 +456,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
         fi[i].map_field = b->map_field + I.map_off;
         fi[i].tA = b->tA + 2 * (size_t)I.map_off;
         fi[i].tB = b->tB + 2 * (size_t)I.map_off;
-        fi[i].ex = b->ex + I.map_off;
-        fi[i].ey = b->ey + I.map_off;
+        fi[i].exy = b->exy + I.map_off;
         fi[i].map_max = b->map_max + i;
         fi[i].map_mean = b->map_mean + i;
         fi[i].skip = &b->d_st[i].done;

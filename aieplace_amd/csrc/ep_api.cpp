@@ -240,7 +240,7 @@ static void destroy_ctx(ep_ctx *c)
     if (c->ctl_exec) (void)hipGraphExecDestroy(c->ctl_exec);
     void *ptrs[] = {c->wh, c->net_start, c->pin_cell, c->pin_off, c->pin_grad, c->cell_pin_start,
                     c->cell_pins, c->V[0], c->V[1], c->U[0], c->U[1], c->G[0], c->G[1],
-                    c->map_field, c->map_mov, c->map_fix, c->acc_field, c->acc_mov, c->ex, c->ey, c->tA, c->tB, c->part,
+                    c->map_field, c->map_mov, c->map_fix, c->acc_field, c->acc_mov, c->exy, c->tA, c->tB, c->part,
                     c->sorted, c->sort_count, c->sort_off, c->dens_tab,
                     c->d_scal, c->d_res, c->d_mlp_w, c->d_ctl, c->d_log, c->d_rec, c->fin,
                     c->fin_ticket};
@@ -560,8 +560,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     EP_TRY(dalloc(&c->sort_count, kSortTiles));
     EP_TRY(dalloc(&c->sort_off, kSortTiles));
     EP_TRY(dalloc(&c->dens_tab, kSortTiles + 1));
-    EP_TRY(dalloc(&c->ex, nb));
-    EP_TRY(dalloc(&c->ey, nb));
+    EP_TRY(dalloc(&c->exy, nb));
     EP_TRY(dalloc(&c->tA, 2 * nb));
     EP_TRY(dalloc(&c->tB, 2 * nb));
     EP_TRY(dalloc(&c->part, c->part_total));
@@ -593,8 +592,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
         }
     }
     if (hipMemset(c->pin_grad, 0, sizeof(float2) * std::max(1, L.n_pins)) != hipSuccess ||
-        hipMemset(c->ex, 0, sizeof(float) * nb) != hipSuccess ||
-        hipMemset(c->ey, 0, sizeof(float) * nb) != hipSuccess ||
+        hipMemset(c->exy, 0, sizeof(float2) * nb) != hipSuccess ||
         hipMemset(c->part, 0, sizeof(double) * c->part_total) != hipSuccess ||
         hipMemset(c->fin_ticket, 0, sizeof(unsigned)) != hipSuccess ||
         hipMemset(c->sort_count, 0, sizeof(int32_t) * kSortTiles) != hipSuccess) {
@@ -753,8 +751,13 @@ extern "C" ep_status ep_get_field(ep_ctx *c, float *ex, float *ey)
 {
     if (!c) return fail(EP_EINVAL, "ep_get_field: null context");
     const size_t nb = (size_t)c->Mx * c->My;
-    if (ex) EP_HIP(hipMemcpy(ex, c->ex, nb * sizeof(float), hipMemcpyDeviceToHost));
-    if (ey) EP_HIP(hipMemcpy(ey, c->ey, nb * sizeof(float), hipMemcpyDeviceToHost));
+    if (!ex && !ey) return EP_OK;
+    std::vector<float2> e(nb);
+    EP_HIP(hipMemcpy(e.data(), c->exy, nb * sizeof(float2), hipMemcpyDeviceToHost));
+    for (size_t b = 0; b < nb; ++b) {
+        if (ex) ex[b] = e[b].x;
+        if (ey) ey[b] = e[b].y;
+    }
     return EP_OK;
 }
 

@@ -2,7 +2,7 @@
 //
 // All instances share concatenated device arrays.  Instance i owns
 //   cells [cell_off, cell_off + n_all)   of wh, V[2], U[2], G[2], sorted
-//   bins  [map_off, map_off + M*M)       of map_field/map_mov/map_fix/ex/ey
+//   bins  [map_off, map_off + M*M)       of map_field/map_mov/map_fix/exy
 //   bins  [2*map_off, 2*map_off + 2*M*M) of the transform scratch tA/tB
 //   nets and pins: contiguous ranges of net_start/pin_cell/pin_off/pin_grad,
 //   with net_start holding global pin indices and pin_cell global cell indices
@@ -83,7 +83,8 @@ struct BDev {
     const float2 *pin_off;
     float2 *pin_grad;
     const int32_t *cps, *cpins;  // global cell -> pin CSR (empty for non-movable cells)
-    float *map_field, *map_mov, *map_fix, *ex, *ey;
+    float *map_field, *map_mov, *map_fix;
+    float2 *exy;                 // field (ex, ey) per bin
     unsigned long long *acc_field, *acc_mov;  // fixed-point scatter targets (2^32 per bin area)
     float *map_mean;             // [n_inst]
     uint32_t *map_max;           // [n_inst] float bits of max |map - mean|
@@ -124,7 +125,8 @@ struct BatchFieldInst {
     int M;
     float bw, bh;
     const float *map_field;
-    float *tA, *tB, *ex, *ey;
+    float *tA, *tB;
+    float2 *exy;
     const uint32_t *map_max;  // float bits of max |map - mean| (split FP16 scaling)
     const float *map_mean;
     const int32_t *skip;      // instance finished

@@ -14,7 +14,9 @@
 //   map_field, map_mov, map_fix [Mx*My] float  density maps (x-major)
 //   acc_field, acc_mov [Mx*My] uint64  scatter accumulators (2^32 per bin area)
 //   sorted[n_place]      float4  cells (x, y, +-w, h) in coarse-tile order
-//   ex, ey [Mx*My]        float  field; spectral / transform scratch
+//   exy[Mx*My]           float2 field (ex, ey) per bin, interleaved: one 8-byte
+//                                gather per bin in the per-cell force
+//   tA, tB                       spectral / transform scratch
 //   transform matrices (FP32 and split BF16/FP16 copies)
 #pragma once
 
@@ -243,7 +245,7 @@ struct Ctx {
     int32_t *sort_count = nullptr, *sort_off = nullptr;  // kSortTiles each
     int32_t *dens_tab = nullptr;                     // kSortTiles + 1: density block table
     DensPlan dens{};
-    float *ex = nullptr, *ey = nullptr;
+    float2 *exy = nullptr;                      // field (ex, ey) per bin
     float *tA = nullptr, *tB = nullptr;         // transform scratch, 2*Mx*My each
     void *split_mats = nullptr;                 // host handle: FP32 + split transform matrices
     uint32_t *field_max = nullptr;              // [4] running max |operand| for split FP16

@@ -599,8 +599,8 @@ __device__ inline float2 sum_pin_partials(int k0, int k1, const int32_t *__restr
 // (ePlace local smoothing, the adjoint of the field-map scatter).  Rows first,
 // the y overlaps shared by every row, so the field gathers of a row issue
 // together.
-__device__ inline float2 cell_force(float2 p, float2 s, const float *__restrict__ ex,
-                                    const float *__restrict__ ey, const GeomArgs &g, int Mx, int My)
+__device__ inline float2 cell_force(float2 p, float2 s, const float2 *__restrict__ exy,
+                                    const GeomArgs &g, int Mx, int My)
 {
     Rect r;
     stretched_rect(p, s, g, Mx, My, r);
@@ -614,18 +614,15 @@ __device__ inline float2 cell_force(float2 p, float2 s, const float *__restrict_
             for (int i = r.i0; i <= r.i1; ++i) {
                 const float ox = ovl(r.tx0, r.tx1, i);
                 const size_t base = (size_t)i * My + r.j0;
-                float vx[4], vy[4];
+                float2 e[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    vx[q] = q <= dj ? ex[base + q] : 0.0f;
-                    vy[q] = q <= dj ? ey[base + q] : 0.0f;
-                }
+                for (int q = 0; q < 4; ++q) e[q] = q <= dj ? exy[base + q] : make_float2(0.0f, 0.0f);
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     if (q <= dj) {
                         const float o = ox * oy[q];
-                        fx += o * vx[q];
-                        fy += o * vy[q];
+                        fx += o * e[q].x;
+                        fy += o * e[q].y;
                     }
             }
         } else {
@@ -634,8 +631,9 @@ __device__ inline float2 cell_force(float2 p, float2 s, const float *__restrict_
                 for (int j = r.j0; j <= r.j1; ++j) {
                     const size_t b = (size_t)i * My + j;
                     const float o = ox * ovl(r.ty0, r.ty1, j);
-                    fx += o * ex[b];
-                    fy += o * ey[b];
+                    const float2 e = exy[b];
+                    fx += o * e.x;
+                    fy += o * e.y;
                 }
             }
         }

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void kb_cell_grad(BDev d, int at)
     const float lambda = S->f_lambda;
     const GeomArgs g = inst_geom(I);
     const int M = I.M;
-    const float *ex = d.ex + I.map_off, *ey = d.ey + I.map_off;
+    const float2 *exy = d.exy + I.map_off;
     const int lc = (lb - d.pre_cell[i]) * kBlock + threadIdx.x;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (lc < I.n_place) {
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void kb_cell_grad(BDev d, int at)
             g0 = pick<const float2 *>(d.G[0], d.G[1], a)[c];
         }
         const int k0 = d.cps[c], k1 = d.cps[c + 1];
-        const float2 f = cell_force(p, s, ex, ey, g, M, M);
+        const float2 f = cell_force(p, s, exy, g, M, M);
         const float2 gw = sum_pin_partials(k0, k1, d.cpins, d.pin_grad);
         const float pre = fmaxf(1.0f, (float)(k1 - k0) + lambda * s.x * s.y);
         const float gx = (gw.x - lambda * f.x) / pre, gy = (gw.y - lambda * f.y) / pre;

@@ -67,6 +67,7 @@ struct GemmArgs {
     uint32_t *cmax;             // FP16X3: running max |output| of this product
     const float *bbias;         // nullable: constant subtracted from the data operand
     const int32_t *skip;        // nonzero: the product is skipped (finished run or instance)
+    int c_pair;                 // 1: C[z] is one half of an interleaved pair map (element o at 2 o)
 };
 
 __device__ inline float spectral_val(const GemmArgs &a, int row, int col, float v, float *o2)
@@ -85,7 +86,7 @@ __device__ inline float store_out(const GemmArgs &a, int z, int epi, int row, in
 {
     const size_t o = (size_t)row * a.N + col;
     if (epi == 0) {
-        a.C[z][o] = v;
+        a.C[z][a.c_pair ? 2 * o : o] = v;
         return fabsf(v);
     }
     float v2;
@@ -508,17 +509,17 @@ __global__ void k_clear_max(uint32_t *m, int n)
 }
 
 // Block partials of max|ex - exd|, max|exd|, max|ey - eyd|, max|eyd|.
-__global__ __launch_bounds__(256) void k_field_err(const float *__restrict__ ex,
-                                                   const float *__restrict__ ey,
+__global__ __launch_bounds__(256) void k_field_err(const float2 *__restrict__ exy,
                                                    const double *__restrict__ exd,
                                                    const double *__restrict__ eyd, int nb,
                                                    double *__restrict__ part)
 {
     double v[4] = {0, 0, 0, 0};
     for (int b = blockIdx.x * 256 + threadIdx.x; b < nb; b += gridDim.x * 256) {
-        v[0] = fmax(v[0], fabs((double)ex[b] - exd[b]));
+        const float2 e = exy[b];
+        v[0] = fmax(v[0], fabs((double)e.x - exd[b]));
         v[1] = fmax(v[1], fabs(exd[b]));
-        v[2] = fmax(v[2], fabs((double)ey[b] - eyd[b]));
+        v[2] = fmax(v[2], fabs((double)e.y - eyd[b]));
         v[3] = fmax(v[3], fabs(eyd[b]));
     }
     __shared__ double red[4][4];
@@ -775,8 +776,9 @@ void launch_field(Ctx &c, int precision, hipStream_t s)
     gemm(a, 2, true, 0, precision, s);
     // G4: Ex[i][j] = SxT * R1t^T, Ey = CxT * R2t^T
     a.M = c.Mx; a.N = c.My; a.K = c.Mx;
-    setA(0, MAT_SXT); a.B[0] = c.tA; a.C[0] = c.ex;
-    setA(1, MAT_CXT); a.B[1] = c.tA + nb; a.C[1] = c.ey;
+    setA(0, MAT_SXT); a.B[0] = c.tA; a.C[0] = reinterpret_cast<float *>(c.exy);
+    setA(1, MAT_CXT); a.B[1] = c.tA + nb; a.C[1] = reinterpret_cast<float *>(c.exy) + 1;
+    a.c_pair = 1;
     a.bmax = fm->fmax + 3; a.cmax = nullptr;
     gemm(a, 2, true, 0, precision, s);
 }
@@ -829,7 +831,7 @@ ep_status field_error(Ctx &c, double *err_ex, double *err_ey)
     a.A[0] = fm->f64[MAT_SXT]; a.Bd[0] = fm->tA64; a.C[0] = fm->exd;
     a.A[1] = fm->f64[MAT_CXT]; a.Bd[1] = fm->tA64 + nb; a.C[1] = fm->eyd;
     run(2, 0);
-    hipLaunchKernelGGL(k_field_err, dim3(kErrBlocks), dim3(256), 0, s, c.ex, c.ey, fm->exd, fm->eyd,
+    hipLaunchKernelGGL(k_field_err, dim3(kErrBlocks), dim3(256), 0, s, c.exy, fm->exd, fm->eyd,
                        (int)nb, fm->err_part);
     if (hipGetLastError() != hipSuccess) return fail(EP_EDEVICE, "field_error: launch failed");
     std::vector<double> part(kErrBlocks * 4);
@@ -923,8 +925,9 @@ ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &
             setA(g3, 1, MAT_SYT); g3.B[1] = d.tB + nb; g3.C[1] = d.tA + nb;
             g3.bmax = fm + 2; g3.cmax = f16 ? fm + 3 : nullptr;
             GemmArgs g4 = a;
-            setA(g4, 0, MAT_SXT); g4.B[0] = d.tA; g4.C[0] = d.ex;
-            setA(g4, 1, MAT_CXT); g4.B[1] = d.tA + nb; g4.C[1] = d.ey;
+            setA(g4, 0, MAT_SXT); g4.B[0] = d.tA; g4.C[0] = reinterpret_cast<float *>(d.exy);
+            setA(g4, 1, MAT_CXT); g4.B[1] = d.tA + nb; g4.C[1] = reinterpret_cast<float *>(d.exy) + 1;
+            g4.c_pair = 1;
             g4.bmax = fm + 3; g4.cmax = nullptr;
             st[0].push_back(g1);
             st[1].push_back(g2);

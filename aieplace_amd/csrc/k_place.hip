@@ -306,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
     Pair<const float2> pos_pair, const float2 *__restrict__ wh, int n_place, int n_mov,
     GeomArgs g, int Mx, int My, const int32_t *__restrict__ cell_pin_start,
     const int32_t *__restrict__ cell_pins, const float2 *__restrict__ pin_grad,
-    const float *__restrict__ ex, const float *__restrict__ ey, Pair<const float2> vref_pair,
+    const float2 *__restrict__ exy, Pair<const float2> vref_pair,
     Pair<const float2> gref_pair, Pair<float2> gout_pair, float2 *__restrict__ gw_out,
     float2 *__restrict__ f_out, const DevScalars *__restrict__ sc, double *__restrict__ part)
 {
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
             k1 = cell_pin_start[c + 1];
         }
         const int np = k1 - k0;
-        const float2 f = cell_force(p, s, ex, ey, g, Mx, My);
+        const float2 f = cell_force(p, s, exy, g, Mx, My);
         const float2 gw = sum_pin_partials(k0, k1, cell_pins, pin_grad);
         const float gwx = gw.x, gwy = gw.y, fx = f.x, fy = f.y;
         const float pre = fmaxf(1.0f, (float)np + lambda * s.x * s.y);
@@ -762,7 +762,7 @@ void launch_cell_grad(Ctx &c, Pair<const float2> pos, Pair<const float2> vref,
 {
     hipLaunchKernelGGL(k_cell_grad, dim3(c.cell_blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
                        c.n_mov, geom_args(c), c.Mx, c.My, c.cell_pin_start, c.cell_pins,
-                       c.pin_grad, c.ex, c.ey, vref, gref, gout, gw_out, f_out, c.d_scal,
+                       c.pin_grad, c.exy, vref, gref, gout, gw_out, f_out, c.d_scal,
                        c.part + c.part_cell_off);
 }
 
