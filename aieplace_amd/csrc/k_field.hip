@@ -171,19 +171,29 @@ __device__ inline float sym_store(const GemmArgs &a, int z, int h0, int n0, cons
 }
 
 // ---------------------------------------------------------------- FP32
+//
+// SK half-k per step (one barrier pair per step; the next step's operands are
+// loaded while the MFMAs run).  The k order of the MFMA chain does not depend
+// on SK, so every SK gives bit-identical results; products on one map use long
+// steps, batched ones shorter steps so that two blocks share a CU.  The
+// staging lives in dynamic LDS (sizeof(StageF32<SK>) bytes per block).
 
-constexpr int SK32 = 32;           // half-k per step
-constexpr int P32 = SK32 / 16;     // load pieces per thread and step
+template <int SK>
+struct StageF32 {
+    float a[2][SK][SBM + 4];     // [E|O][k][half-row]
+    float b[2][SK][SBN + 4];     // [B+|B- or Be|Bo][k][column]
+};
 
-template <bool INV, int EPI>
+template <bool INV, int EPI, int SK>
 __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
 {
+    constexpr int P = SK / 16;  // load pieces per thread and step
+    extern __shared__ __attribute__((aligned(16))) float gemm_lds[];
+    StageF32<SK> &S = *reinterpret_cast<StageF32<SK> *>(gemm_lds);
     const int N = a.N, K = a.K, Mh = a.M >> 1, Kh = K >> 1;
     const float *__restrict__ Ae = a.A[z];
     const float *__restrict__ Ao = a.Ao[z];
     const float *__restrict__ B = a.B[z];
-    __shared__ float sA[2][SK32][SBM + 4];
-    __shared__ float sB[2][SK32][SBN + 4];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int eo = wv >> 1, wm = (wv & 1) * 32;
     const int ar = t >> 2, ak = (t & 3) * 4;  // A: 64 rows x 16 k per piece, 4 per thread and half
@@ -193,11 +203,11 @@ __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-    float4 ae[P32], ao[P32];
-    float2 b0[P32], b1[P32];
+    float4 ae[P], ao[P];
+    float2 b0[P], b1[P];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < P32; ++q) {
+        for (int q = 0; q < P; ++q) {
             const int ka = k0 + q * 16 + ak, kb = k0 + q * 16 + bk;
             ae[q] = ao[q] = make_float4(0, 0, 0, 0);
             b0[q] = b1[q] = make_float2(0, 0);
@@ -219,49 +229,55 @@ __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
         }
     };
     load(0);
-    for (int k0 = 0; k0 < Kh; k0 += SK32) {
+    for (int k0 = 0; k0 < Kh; k0 += SK) {
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < P32; ++q) {
+        for (int q = 0; q < P; ++q) {
             const int ka = q * 16 + ak, kb = q * 16 + bk;
-            sA[0][ka + 0][ar] = ae[q].x; sA[0][ka + 1][ar] = ae[q].y;
-            sA[0][ka + 2][ar] = ae[q].z; sA[0][ka + 3][ar] = ae[q].w;
-            sA[1][ka + 0][ar] = ao[q].x; sA[1][ka + 1][ar] = ao[q].y;
-            sA[1][ka + 2][ar] = ao[q].z; sA[1][ka + 3][ar] = ao[q].w;
+            S.a[0][ka + 0][ar] = ae[q].x; S.a[0][ka + 1][ar] = ae[q].y;
+            S.a[0][ka + 2][ar] = ae[q].z; S.a[0][ka + 3][ar] = ae[q].w;
+            S.a[1][ka + 0][ar] = ao[q].x; S.a[1][ka + 1][ar] = ao[q].y;
+            S.a[1][ka + 2][ar] = ao[q].z; S.a[1][ka + 3][ar] = ao[q].w;
             if (INV) {
-                sB[0][kb][br] = b0[q].x; sB[0][kb + 1][br] = b0[q].y;
-                sB[1][kb][br] = b1[q].x; sB[1][kb + 1][br] = b1[q].y;
+                S.b[0][kb][br] = b0[q].x; S.b[0][kb + 1][br] = b0[q].y;
+                S.b[1][kb][br] = b1[q].x; S.b[1][kb + 1][br] = b1[q].y;
             } else {
                 const float d0 = b0[q].x - bias, d1 = b0[q].y - bias;  // b[k], b[k+1]
                 const float m0 = b1[q].y - bias, m1 = b1[q].x - bias;  // b[K-1-k], b[K-2-k]
-                sB[0][kb][br] = d0 + m0; sB[0][kb + 1][br] = d1 + m1;
-                sB[1][kb][br] = d0 - m0; sB[1][kb + 1][br] = d1 - m1;
+                S.b[0][kb][br] = d0 + m0; S.b[0][kb + 1][br] = d1 + m1;
+                S.b[1][kb][br] = d0 - m0; S.b[1][kb + 1][br] = d1 - m1;
             }
         }
         __syncthreads();
-        if (k0 + SK32 < Kh) load(k0 + SK32);  // next tiles in flight during the MFMAs
+        if (k0 + SK < Kh) load(k0 + SK);  // next tiles in flight during the MFMAs
 #pragma unroll
-        for (int kk = 0; kk < SK32; kk += 2) {
-            const float x = sA[eo][kk + (lane >> 5)][wm + (lane & 31)];
-            const float y = sB[eo][kk + (lane >> 5)][lane & 31];
+        for (int kk = 0; kk < SK; kk += 2) {
+            const float x = S.a[eo][kk + (lane >> 5)][wm + (lane & 31)];
+            const float y = S.b[eo][kk + (lane >> 5)][lane & 31];
             acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x, y, acc, 0, 0, 0);
         }
     }
-    sym_store<INV, EPI>(a, z, h0, n0, acc, 1.0f, &sA[0][0][0]);
+    sym_store<INV, EPI>(a, z, h0, n0, acc, 1.0f, &S.a[0][0][0]);
 }
 
-template <bool INV, int EPI>
+template <bool INV, int EPI, int SK>
 __global__ __launch_bounds__(256) void k_sym_f32(GemmArgs a)
 {
     if (a.skip && *a.skip) return;
-    sym_f32_tile<INV, EPI>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
+    sym_f32_tile<INV, EPI, SK>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
 // ---------------------------------------------------------------- split 16-bit
 
-constexpr int SK16 = 64;          // half-k per step (four 32x32x16 MFMA sub-steps)
-constexpr int P16 = SK16 / 32;    // load pieces per thread and step
-constexpr int SLDK = SK16 + 8;    // padded LDS row (16-bit elements)
+// SK half-k per step (SK / 16 sub-steps of three 32x32x16 MFMAs), staged in
+// dynamic LDS (sizeof(StageSplit<SK>) bytes per block); as in the FP32 form
+// the result does not depend on SK.
+template <int SK>
+struct StageSplit {
+    static constexpr int LD = SK + 8;  // padded LDS row (16-bit elements)
+    uint16_t ah[2][SBM][LD], al[2][SBM][LD];  // [E|O][half-row][k], hi and lo planes
+    uint16_t bh[2][SBN][LD], bl[2][SBN][LD];  // [B+|B- or Be|Bo][column][k]
+};
 
 __device__ inline uint16_t bf16_rne(float x)
 {
@@ -306,19 +322,18 @@ __device__ inline void split_store4(const float (&v)[4], uint16_t *dh, uint16_t 
     *reinterpret_cast<uint2 *>(dl) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
 }
 
-template <bool INV, int EPI, bool F16>
+template <bool INV, int EPI, bool F16, int SK>
 __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
 {
+    constexpr int P16 = SK / 32;  // load pieces per thread and step
+    extern __shared__ __attribute__((aligned(16))) float gemm_lds[];
+    StageSplit<SK> &S = *reinterpret_cast<StageSplit<SK> *>(gemm_lds);
     const int N = a.N, K = a.K, Mh = a.M >> 1, Kh = K >> 1;
     const uint16_t *__restrict__ Aeh = a.Ah[z];
     const uint16_t *__restrict__ Ael = a.Al[z];
     const uint16_t *__restrict__ Aoh = a.Aoh[z];
     const uint16_t *__restrict__ Aol = a.Aol[z];
     const float *__restrict__ B = a.B[z];
-    __shared__ __attribute__((aligned(16))) uint16_t sAh[2][SBM][SLDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sAl[2][SBM][SLDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sBh[2][SBN][SLDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sBl[2][SBN][SLDK];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int eo = wv >> 1, wm = (wv & 1) * 32;
     const int ar = t >> 2, ak = (t & 3) * 8;  // A: 64 rows x 32 half-k per piece, 8 per thread and plane
@@ -366,15 +381,15 @@ __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
         }
     };
     load(0);
-    for (int k0 = 0; k0 < Kh; k0 += SK16) {
+    for (int k0 = 0; k0 < Kh; k0 += SK) {
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < P16; ++q) {
             const int ka = q * 32 + ak, kb = q * 32 + bk;
-            *reinterpret_cast<uint4 *>(&sAh[0][ar][ka]) = aeh[q];
-            *reinterpret_cast<uint4 *>(&sAl[0][ar][ka]) = ael[q];
-            *reinterpret_cast<uint4 *>(&sAh[1][ar][ka]) = aoh[q];
-            *reinterpret_cast<uint4 *>(&sAl[1][ar][ka]) = aol[q];
+            *reinterpret_cast<uint4 *>(&S.ah[0][ar][ka]) = aeh[q];
+            *reinterpret_cast<uint4 *>(&S.al[0][ar][ka]) = ael[q];
+            *reinterpret_cast<uint4 *>(&S.ah[1][ar][ka]) = aoh[q];
+            *reinterpret_cast<uint4 *>(&S.al[1][ar][ka]) = aol[q];
             float p[4], m[4];
             if (INV) {
                 p[0] = b0[q].x; p[1] = b0[q].y; p[2] = b0[q].z; p[3] = b0[q].w;
@@ -395,53 +410,53 @@ __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
                     m[j] *= bscale;
                 }
             }
-            split_store4<F16>(p, &sBh[0][br][kb], &sBl[0][br][kb]);
-            split_store4<F16>(m, &sBh[1][br][kb], &sBl[1][br][kb]);
+            split_store4<F16>(p, &S.bh[0][br][kb], &S.bl[0][br][kb]);
+            split_store4<F16>(m, &S.bh[1][br][kb], &S.bl[1][br][kb]);
         }
         __syncthreads();
-        if (k0 + SK16 < Kh) load(k0 + SK16);
+        if (k0 + SK < Kh) load(k0 + SK);
 #pragma unroll
-        for (int ks = 0; ks < SK16; ks += 16) {
+        for (int ks = 0; ks < SK; ks += 16) {
             const int kk = ks + 8 * (lane >> 5);
-            const uint16_t *xh = &sAh[eo][wm + (lane & 31)][kk], *xl = &sAl[eo][wm + (lane & 31)][kk];
-            const uint16_t *yh = &sBh[eo][lane & 31][kk], *yl = &sBl[eo][lane & 31][kk];
+            const uint16_t *xh = &S.ah[eo][wm + (lane & 31)][kk], *xl = &S.al[eo][wm + (lane & 31)][kk];
+            const uint16_t *yh = &S.bh[eo][lane & 31][kk], *yl = &S.bl[eo][lane & 31][kk];
             acc = mfma16<F16>(xh, yh, acc);
             acc = mfma16<F16>(xh, yl, acc);
             acc = mfma16<F16>(xl, yh, acc);
         }
     }
     const float m = sym_store<INV, EPI>(a, z, h0, n0, acc, 1.0f / bscale,
-                                        reinterpret_cast<float *>(&sAh[0][0][0]));
+                                        reinterpret_cast<float *>(&S.ah[0][0][0]));
     if (F16 && a.cmax) track_max(a.cmax, m);
 }
 
-template <bool INV, int EPI, bool F16>
+template <bool INV, int EPI, bool F16, int SK>
 __global__ __launch_bounds__(256) void k_sym_split(GemmArgs a)
 {
     if (a.skip && *a.skip) return;
-    sym_split_tile<INV, EPI, F16>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
+    sym_split_tile<INV, EPI, F16, SK>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
 // Batched products: block b runs tile tiles[b] = (instance, h0, n0, z) with the
 // instance's argument block; finished instances are skipped (uniform per block).
-template <bool INV, int EPI>
+template <bool INV, int EPI, int SK>
 __global__ __launch_bounds__(256) void k_sym_f32_b(const GemmArgs *__restrict__ args,
                                                    const int4 *__restrict__ tiles)
 {
     const int4 t = tiles[blockIdx.x];
     const GemmArgs &a = args[t.x];
     if (a.skip && *a.skip) return;
-    sym_f32_tile<INV, EPI>(a, t.w, t.y, t.z);
+    sym_f32_tile<INV, EPI, SK>(a, t.w, t.y, t.z);
 }
 
-template <bool INV, int EPI, bool F16>
+template <bool INV, int EPI, bool F16, int SK>
 __global__ __launch_bounds__(256) void k_sym_split_b(const GemmArgs *__restrict__ args,
                                                      const int4 *__restrict__ tiles)
 {
     const int4 t = tiles[blockIdx.x];
     const GemmArgs &a = args[t.x];
     if (a.skip && *a.skip) return;
-    sym_split_tile<INV, EPI, F16>(a, t.w, t.y, t.z);
+    sym_split_tile<INV, EPI, F16, SK>(a, t.w, t.y, t.z);
 }
 
 // ---------------------------------------------------------------- FP64 reference
@@ -535,23 +550,43 @@ __global__ __launch_bounds__(256) void k_field_err(const float2 *__restrict__ ex
     }
 }
 
+// K steps of the products: long steps for a single map (one block per CU
+// anyway), shorter ones when two maps run so that two blocks share a CU.
+constexpr int kSkF32Single = 32, kSkF32Batch = 32;
+constexpr int kSkSplitSingle = 128, kSkSplitBatch = 64;
+
+template <int SK>
+void gemm_f32(const GemmArgs &a, dim3 grid, bool inv, int epi, hipStream_t s)
+{
+    const size_t lds = sizeof(StageF32<SK>);
+    if (inv) hipLaunchKernelGGL((k_sym_f32<true, 0, SK>), grid, dim3(256), lds, s, a);
+    else if (epi) hipLaunchKernelGGL((k_sym_f32<false, 1, SK>), grid, dim3(256), lds, s, a);
+    else hipLaunchKernelGGL((k_sym_f32<false, 0, SK>), grid, dim3(256), lds, s, a);
+}
+
+template <bool F16, int SK>
+void gemm_split(const GemmArgs &a, dim3 grid, bool inv, int epi, hipStream_t s)
+{
+    const size_t lds = sizeof(StageSplit<SK>);
+    if (inv) hipLaunchKernelGGL((k_sym_split<true, 0, F16, SK>), grid, dim3(256), lds, s, a);
+    else if (epi) hipLaunchKernelGGL((k_sym_split<false, 1, F16, SK>), grid, dim3(256), lds, s, a);
+    else hipLaunchKernelGGL((k_sym_split<false, 0, F16, SK>), grid, dim3(256), lds, s, a);
+}
+
 // One product (or `batch` independent ones) of the field solve.
 void gemm(const GemmArgs &a, int batch, bool inv, int epi, int precision, hipStream_t s)
 {
     const dim3 grid((a.N + SBN - 1) / SBN, ((a.M >> 1) + SBM - 1) / SBM, batch);
-    const dim3 blk(256);
+    const bool single = batch == 1;
     if (precision == EP_PREC_FP32) {
-        if (inv) hipLaunchKernelGGL((k_sym_f32<true, 0>), grid, blk, 0, s, a);
-        else if (epi) hipLaunchKernelGGL((k_sym_f32<false, 1>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((k_sym_f32<false, 0>), grid, blk, 0, s, a);
+        if (single) gemm_f32<kSkF32Single>(a, grid, inv, epi, s);
+        else gemm_f32<kSkF32Batch>(a, grid, inv, epi, s);
     } else if (precision == EP_PREC_FP16X3) {
-        if (inv) hipLaunchKernelGGL((k_sym_split<true, 0, true>), grid, blk, 0, s, a);
-        else if (epi) hipLaunchKernelGGL((k_sym_split<false, 1, true>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((k_sym_split<false, 0, true>), grid, blk, 0, s, a);
+        if (single) gemm_split<true, kSkSplitSingle>(a, grid, inv, epi, s);
+        else gemm_split<true, kSkSplitBatch>(a, grid, inv, epi, s);
     } else {
-        if (inv) hipLaunchKernelGGL((k_sym_split<true, 0, false>), grid, blk, 0, s, a);
-        else if (epi) hipLaunchKernelGGL((k_sym_split<false, 1, false>), grid, blk, 0, s, a);
-        else hipLaunchKernelGGL((k_sym_split<false, 0, false>), grid, blk, 0, s, a);
+        if (single) gemm_split<false, kSkSplitSingle>(a, grid, inv, epi, s);
+        else gemm_split<false, kSkSplitBatch>(a, grid, inv, epi, s);
     }
 }
 
@@ -982,18 +1017,23 @@ void launch_field_batch(BatchField *bf, int precision, hipStream_t s)
         const GemmArgs *args = bf->d_args[pi][stage];
         const int4 *tiles = bf->d_tiles[stage];
         const bool inv = stage >= 2, epi = stage == 1;
+        constexpr int SF = kSkF32Batch, SS = kSkSplitBatch;
         if (precision == EP_PREC_FP32) {
-            if (inv) hipLaunchKernelGGL((k_sym_f32_b<true, 0>), grid, blk, 0, s, args, tiles);
-            else if (epi) hipLaunchKernelGGL((k_sym_f32_b<false, 1>), grid, blk, 0, s, args, tiles);
-            else hipLaunchKernelGGL((k_sym_f32_b<false, 0>), grid, blk, 0, s, args, tiles);
-        } else if (precision == EP_PREC_BF16X3) {
-            if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, false>), grid, blk, 0, s, args, tiles);
-            else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, false>), grid, blk, 0, s, args, tiles);
-            else hipLaunchKernelGGL((k_sym_split_b<false, 0, false>), grid, blk, 0, s, args, tiles);
+            const size_t lds = sizeof(StageF32<SF>);
+            if (inv) hipLaunchKernelGGL((k_sym_f32_b<true, 0, SF>), grid, blk, lds, s, args, tiles);
+            else if (epi) hipLaunchKernelGGL((k_sym_f32_b<false, 1, SF>), grid, blk, lds, s, args, tiles);
+            else hipLaunchKernelGGL((k_sym_f32_b<false, 0, SF>), grid, blk, lds, s, args, tiles);
         } else {
-            if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, true>), grid, blk, 0, s, args, tiles);
-            else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, true>), grid, blk, 0, s, args, tiles);
-            else hipLaunchKernelGGL((k_sym_split_b<false, 0, true>), grid, blk, 0, s, args, tiles);
+            const size_t lds = sizeof(StageSplit<SS>);
+            if (precision == EP_PREC_BF16X3) {
+                if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, false, SS>), grid, blk, lds, s, args, tiles);
+                else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, false, SS>), grid, blk, lds, s, args, tiles);
+                else hipLaunchKernelGGL((k_sym_split_b<false, 0, false, SS>), grid, blk, lds, s, args, tiles);
+            } else {
+                if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, true, SS>), grid, blk, lds, s, args, tiles);
+                else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, true, SS>), grid, blk, lds, s, args, tiles);
+                else hipLaunchKernelGGL((k_sym_split_b<false, 0, true, SS>), grid, blk, lds, s, args, tiles);
+            }
         }
     }
 }
