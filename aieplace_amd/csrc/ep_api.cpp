@@ -92,7 +92,6 @@ void push_scalars(Ctx &c, hipStream_t s)
 void enqueue_eval(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref, float2 *gout,
                   float2 *gw_out, float2 *f_out, hipStream_t s, bool counted = false)
 {
-    launch_fill_maps(c, s);
     launch_sort(c, pos, counted, s);
     launch_density(c, s);
     launch_overflow(c, s);
@@ -575,6 +574,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     }
     std::memset(c->h_scal, 0, sizeof(DevScalars));
     std::memset(c->h_res, 0, sizeof(DevResult));
+    c->h_scal->step_unit = c->bw;
     EP_TRY(upload(c->d_scal, c->h_scal, 1));  // sel = halt = 0 before the first push
     EP_TRY(upload(c->wh, L.wh.data(), L.wh.size()));
     EP_TRY(upload(c->net_start, L.net_start.data(), L.net_start.size()));
@@ -595,6 +595,8 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
         hipMemset(c->exy, 0, sizeof(float2) * nb) != hipSuccess ||
         hipMemset(c->part, 0, sizeof(double) * c->part_total) != hipSuccess ||
         hipMemset(c->fin_ticket, 0, sizeof(unsigned)) != hipSuccess ||
+        hipMemset(c->acc_field, 0, sizeof(unsigned long long) * nb) != hipSuccess ||
+        hipMemset(c->acc_mov, 0, sizeof(unsigned long long) * nb) != hipSuccess ||
         hipMemset(c->sort_count, 0, sizeof(int32_t) * kSortTiles) != hipSuccess) {
         destroy_ctx(c);
         return fail(EP_EDEVICE, "ep_create: hipMemset failed");
@@ -696,7 +698,6 @@ extern "C" ep_status ep_get_positions(ep_ctx *c, float *x, float *y)
 
 static ep_status run_density(ep_ctx *c)
 {
-    launch_fill_maps(*c, c->stream);
     launch_sort(*c, c->V[c->cur], false, c->stream);
     launch_density(*c, c->stream);
     launch_overflow(*c, c->stream);
@@ -899,6 +900,7 @@ extern "C" ep_status ep_init_optimizer(ep_ctx *c)
     c->evals++;
     const double sw = c->h_res->sum_gw, sf = c->h_res->sum_f;
     c->lambda = sf > 0 ? (double)c->prm.lambda_eta * sw / sf : (double)c->prm.lambda_eta;
+    c->lambda0 = c->lambda;
     c->prev_hpwl = c->h_res->hpwl;
     c->last_hpwl = c->h_res->hpwl;
     c->last_tau = c->h_res->tau;
@@ -938,24 +940,30 @@ extern "C" ep_status ep_init_optimizer(ep_ctx *c)
     return EP_OK;
 }
 
+// 1 / sqrt(placeable cells): turns the FP64 norms into per-cell RMS values.
+static double inv_sqrt_np(const ep_ctx *c) { return 1.0 / std::sqrt((double)std::max(1, c->n_place)); }
+
 // Step-model features of the coming iteration (layout in include/eplace_mlp.h).
+// Lengths are in bin widths and norms per cell, so instances of any size and
+// grid share one feature scale; k_place.hip's ctl_features is the same.
 static void step_features(const ep_ctx *c, double coef, float *f)
 {
     auto lg = [](double v) { return (float)sched_log10(std::max(v, 1e-30)); };
-    f[0] = lg(c->alpha);
-    f[1] = lg(c->h_alpha1);
-    f[2] = lg(c->h_alpha2);
+    const double bw = c->bw, isn = inv_sqrt_np(c);
+    f[0] = lg(c->alpha / bw);
+    f[1] = lg(c->h_alpha1 / bw);
+    f[2] = lg(c->h_alpha2 / bw);
     f[3] = (float)c->last_tau;
     f[4] = (float)(c->last_tau - c->h_prev_tau);
-    f[5] = lg(c->lambda);
-    f[6] = lg(c->gamma / c->bw);
+    f[5] = lg(c->lambda / c->lambda0);
+    f[6] = lg(c->gamma / bw);
     f[7] = (float)((c->last_hpwl - c->h_prev_hpwl) / std::max(c->last_hpwl, 1e-30));
-    f[8] = lg(c->h_gnorm);
-    f[9] = lg(c->h_dv);
-    f[10] = lg(c->h_dg);
+    f[8] = lg(c->h_gnorm * isn);
+    f[9] = lg(c->h_dv * isn / bw);
+    f[10] = lg(c->h_dg * isn);
     f[11] = (float)coef;
     f[12] = (float)c->h_evals;
-    f[13] = lg(c->h_sum_ratio);
+    f[13] = lg(c->lambda / c->h_sum_ratio);
     f[14] = lg(c->h_ratio);
     f[15] = (float)(c->iteration / 1000.0);
 }
@@ -996,7 +1004,10 @@ extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
     int trials = 0;
     bool accepted = false, learned_used = false, learned_ok = false;
     // learned proposal first, verified by the backtracking condition
-    if (c->prm.learned_step && c->mlp_loaded) {
+    const bool learned = c->prm.learned_step && c->mlp_loaded;
+    const bool propose = learned && c->model_skip == 0;
+    if (learned && !propose) c->model_skip--;
+    if (propose) {
         learned_used = true;
         c->learned_proposals++;
         if ((st = one_trial(c, c->alpha, coef, 1, &alpha_hat, &used)) != EP_OK) return st;
@@ -1004,6 +1015,10 @@ extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
         if (std::isfinite(alpha_hat) && alpha_hat >= 0.95 * used) {
             accepted = learned_ok = true;
             c->learned_accepted++;
+            c->model_backoff = 0;
+        } else {  // the model sits out 2, 4, ... kModelBackoffMax iterations
+            c->model_backoff = c->model_backoff ? std::min(2 * c->model_backoff, kModelBackoffMax) : 2;
+            c->model_skip = c->model_backoff;
         }
     }
     // plain backtracking search from the Lipschitz step (also the fallback)
@@ -1017,7 +1032,7 @@ extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
     // record (features, accepted step) for training the step model
     if (c->rec_target.size() < kMaxStepRecords) {
         c->rec_feat.insert(c->rec_feat.end(), feat, feat + EP_MLP_IN);
-        c->rec_target.push_back((float)sched_log10(std::max(used, 1e-30)));
+        c->rec_target.push_back((float)sched_log10(std::max(used / c->bw, 1e-30)));
     }
     // accept: the trial buffers become the current ones
     c->cur ^= 1;
@@ -1087,6 +1102,7 @@ static CtlParams ctl_params(const ep_ctx *c)
     p.log_cap = c->ctl_cap;
     p.rec_cap = c->ctl_cap;
     p.bw = c->bw;
+    p.inv_sqrt_np = inv_sqrt_np(c);
     return p;
 }
 
@@ -1123,7 +1139,6 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
     for (int t = 0; t < kCtlTrialsPerGraph; ++t) {
         launch_mlp_propose(x, s);
         launch_trial(x, vc, uc, gc, vn, un, true, s);
-        launch_fill_maps(x, s);
         launch_sort(x, vt, true, s);
         launch_density(x, s);
         launch_overflow(x, s);
@@ -1146,7 +1161,7 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
 static void ctl_upload_state(const ep_ctx *c, DevCtl &S)
 {
     std::memset(&S, 0, sizeof(S));
-    S.alpha = c->alpha; S.ak = c->ak; S.lambda = c->lambda; S.gamma = c->gamma;
+    S.alpha = c->alpha; S.ak = c->ak; S.lambda = c->lambda; S.lambda0 = c->lambda0; S.gamma = c->gamma;
     S.prev_hpwl = c->prev_hpwl; S.last_hpwl = c->last_hpwl; S.last_tau = c->last_tau;
     S.last_wa = c->last_wa;
     S.h_alpha1 = c->h_alpha1; S.h_alpha2 = c->h_alpha2; S.h_prev_tau = c->h_prev_tau;
@@ -1154,6 +1169,7 @@ static void ctl_upload_state(const ep_ctx *c, DevCtl &S)
     S.h_dv = c->h_dv; S.h_dg = c->h_dg; S.h_sum_ratio = c->h_sum_ratio;
     S.evals = c->evals; S.iteration = c->iteration; S.h_evals = c->h_evals;
     S.learned_proposals = c->learned_proposals; S.learned_accepted = c->learned_accepted;
+    S.model_skip = c->model_skip; S.model_backoff = c->model_backoff;
     S.cur = c->cur;
 }
 
@@ -1167,6 +1183,7 @@ static void ctl_download_state(ep_ctx *c, const DevCtl &S)
     c->h_dv = S.h_dv; c->h_dg = S.h_dg; c->h_sum_ratio = S.h_sum_ratio;
     c->evals = S.evals; c->iteration = S.iteration; c->h_evals = S.h_evals;
     c->learned_proposals = S.learned_proposals; c->learned_accepted = S.learned_accepted;
+    c->model_skip = S.model_skip; c->model_backoff = S.model_backoff;
     c->cur = S.cur;
 }
 
@@ -1264,7 +1281,6 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
     c->h_scal->coef = 0.0f;
     push_scalars(*c, c->stream);
     // field input must be a real map: build it once at the current positions
-    launch_fill_maps(*c, c->stream);
     launch_sort(*c, c->V[a], false, c->stream);
     launch_density(*c, c->stream);
     launch_overflow(*c, c->stream);
@@ -1288,15 +1304,15 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
         {"cell_sort", 3, 2 * 16.0 * np + 16.0 * np + 8.0 * kSortTiles, 0},
         // packed cells read; two 64-bit accumulator maps flushed
         {"density_scatter", 1, 16.0 * np + 16.0 * nb, 0},
-        // two accumulator maps and the fixed map read, two FP32 maps written
-        {"overflow", 1, 28.0 * nb, 0},
+        // two accumulator maps and the fixed map read; two FP32 maps written, the
+        // accumulators cleared
+        {"overflow", 1, 44.0 * nb, 0},
         {"wa_grad", 1, 4.0 * (N + 1) + 12.0 * P + 8.0 * c->n_all + 8.0 * P, 0},
         {"field_transforms", 4, field_bytes, gemm_flops},
         {"cell_grad", 1, 36.0 * np + 12.0 * pm + 4.0 * c->n_mov + 8.0 * nb, 0},
         // v, u, g, size read; v, u written (the tile counts stay in LDS)
         {"nesterov_trial", 1, 40.0 * np, 0},
         {"finalize", 1, 8.0 * c->part_total, 0},
-        {"fill_maps", 1, 16.0 * nb, 0},
     };
     int k = 0;
     for (const Item &it : items) {
@@ -1310,8 +1326,7 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
             else if (nm == "field_transforms") launch_field(*c, c->prm.precision, c->stream);
             else if (nm == "cell_grad") launch_cell_grad(*c, c->V[a], c->V[a], c->G[a], c->G[b], nullptr, nullptr, c->stream);
             else if (nm == "nesterov_trial") launch_trial(*c, c->V[a], c->U[a], c->G[a], c->V[b], c->U[b], true, c->stream);
-            else if (nm == "finalize") launch_finalize(*c, c->stream);
-            else launch_fill_maps(*c, c->stream);
+            else launch_finalize(*c, c->stream);
         }
         EP_HIP(hipEventRecord(e1, c->stream));
         // the counting trials left tile counts behind: consume them (the scan

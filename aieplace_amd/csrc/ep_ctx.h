@@ -12,7 +12,9 @@
 //   pin_grad[P]          float2  per-pin WA partials (summed per cell by CSR)
 //   cell_pin_start[n_mov+1], cell_pins[P_mov]   cell -> pin CSR
 //   map_field, map_mov, map_fix [Mx*My] float  density maps (x-major)
-//   acc_field, acc_mov [Mx*My] uint64  scatter accumulators (2^32 per bin area)
+//   acc_field, acc_mov [Mx*My] uint64  scatter accumulators (2^32 per bin area);
+//                                zero between evaluations (the overflow pass
+//                                clears what it converts)
 //   sorted[n_place]      float4  cells (x, y, +-w, h) in coarse-tile order
 //   exy[Mx*My]           float2 field (ex, ey) per bin, interleaved: one 8-byte
 //                                gather per bin in the per-cell force
@@ -37,6 +39,10 @@ constexpr int kCellGridCap = 16384;
 constexpr int kSortSide = 64;
 constexpr int kSortTiles = kSortSide * kSortSide;
 constexpr int kSortCellsPerThread = 8;
+// After a rejected model proposal the model sits out 2, 4, ... up to this many
+// iterations (reset by an accepted proposal): phases where it mispredicts cost
+// one extra evaluation per rejection, so they are skipped.
+constexpr int kModelBackoffMax = 32;
 // Step-model records kept per context between ep_take_step_records calls.
 constexpr size_t kMaxStepRecords = (size_t)1 << 21;
 // Default HPWL reference of the lambda schedule: this many bin widths per net.
@@ -113,6 +119,7 @@ struct DevScalars {
     int32_t use_model;  // 1: the MLP proposal kernel overwrites alpha
     int32_t sel;        // index into the Pair arguments of a launch (= cur on device)
     int32_t halt;       // 1: every iteration kernel returns at once (run finished)
+    float step_unit;    // bin width: the step model predicts log10(step / step_unit)
     float feat[16];     // step-model features of the current iteration
 };
 
@@ -134,12 +141,12 @@ inline Pair<T> same(T *x)
 // Optimiser state of a device-controlled run (ep_run with graphs): the host
 // fields of Ctx below, advanced by k_step_ctl after every trial.
 struct DevCtl {
-    double alpha, ak, ak1, coef, lambda, gamma, prev_hpwl;
+    double alpha, ak, ak1, coef, lambda, lambda0, gamma, prev_hpwl;
     double last_hpwl, last_tau, last_wa;
     double h_alpha1, h_alpha2, h_prev_tau, h_prev_hpwl, h_ratio, h_gnorm, h_dv, h_dg, h_sum_ratio;
     int64_t evals;
     int32_t iteration, h_evals, trials, phase, t, learned_used, learned_ok;
-    int32_t learned_proposals, learned_accepted;
+    int32_t learned_proposals, learned_accepted, model_skip, model_backoff;
     int32_t cur, done, error, n_log, n_rec;
     float feat[16];
 };
@@ -149,6 +156,7 @@ struct CtlParams {
     double mu_min, mu_max, hpwl_ref, stop_overflow, gamma_base;  // gamma_base = gamma_bins * bw
     int32_t gamma_schedule, max_iters, max_backtrack, learned, log_cap, rec_cap;
     double bw;
+    double inv_sqrt_np;  // 1 / sqrt(movable + filler cells): per-cell RMS of the norms
 };
 
 // Reductions of one gradient evaluation (FP64), written by the finalize kernel.
@@ -165,6 +173,8 @@ struct DevResult {
 struct Ctx;
 
 // ---- kernel launchers (defined in the .hip files) ----
+// Clears the scatter accumulators (set-up and re-synchronisation; every
+// evaluation's overflow pass leaves them clear).
 void launch_fill_maps(Ctx &c, hipStream_t s);
 // Tile-sorts the cells at pos; `counted`: the tile counts were already made by
 // a counting launch_trial at the same positions.
@@ -263,7 +273,7 @@ struct Ctx {
 
     // optimiser bookkeeping (host)
     bool opt_ready = false;
-    double alpha = 0, ak = 1, lambda = 0, gamma = 0, prev_hpwl = 0;
+    double alpha = 0, ak = 1, lambda = 0, lambda0 = 0, gamma = 0, prev_hpwl = 0;
     double last_hpwl = 0, last_tau = 0, last_wa = 0;
     int32_t iteration = 0;
     int64_t evals = 0;
@@ -287,6 +297,7 @@ struct Ctx {
     double h_gnorm = 0, h_dv = 0, h_dg = 0, h_sum_ratio = 1;
     int32_t h_evals = 1;
     int32_t learned_proposals = 0, learned_accepted = 0;
+    int32_t model_skip = 0, model_backoff = 0;  // rejection backoff of the model
 };
 
 }  // namespace ep

@@ -175,6 +175,14 @@ __device__ inline void block_reduce(double (&v)[NV], double *out)
     }
 }
 
+// Agent-scope load of a double written by another block of the same launch
+// (after that block's release fence and an atomic ticket).
+__device__ inline double load_agent(const double *p)
+{
+    return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
 // ------------------------------------------------------------------ maps
 
 __device__ inline int morton6(int i, int j)

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(MH) void k_mlp_propose(const float *__restrict__ w,
     if (j == 0) {
         float o = w[OFF_B3];
         for (int k = 0; k < MH; ++k) o += w[OFF_W3 + k] * a2[k];
-        const float alpha = exp10f(o * w[33] + w[32]);
+        const float alpha = exp10f(o * w[33] + w[32]) * sc->step_unit;
         if (isfinite(alpha) && alpha > 0.0f) sc->alpha = alpha;  // else keep the Lipschitz step
     }
 }

@@ -146,12 +146,12 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4 *__restrict__ s
 
 // Converts the fixed-point accumulators into the FP32 maps (field map =
 // rho_t * fixed + smoothed cells; movable map = exact movable overlap) and
-// reduces per block the overflow numerator sum_b max(0, mov_b + rho_t*fix_b -
-// rho_t*A_b) and the field map's sum, max and min (the transforms subtract the
-// map mean; split FP16 scales by the centred range).
-// part layout per block: [ovf, sum, max, -min].
-__global__ __launch_bounds__(kBlock) void k_overflow(const unsigned long long *__restrict__ acc_field,
-                                                     const unsigned long long *__restrict__ acc_mov,
+// clears them for the next scatter.  Reduces per block the overflow numerator
+// sum_b max(0, mov_b + rho_t*fix_b - rho_t*A_b) and the field map's sum, max
+// and min (the transforms subtract the map mean; split FP16 scales by the
+// centred range).  part layout per block: [ovf, sum, max, -min].
+__global__ __launch_bounds__(kBlock) void k_overflow(unsigned long long *__restrict__ acc_field,
+                                                     unsigned long long *__restrict__ acc_mov,
                                                      const float *__restrict__ map_fix, double unit,
                                                      float rho_t, float cap, int nb,
                                                      float *__restrict__ map_field,
@@ -163,8 +163,11 @@ __global__ __launch_bounds__(kBlock) void k_overflow(const unsigned long long *_
     double acc[4] = {0.0, 0.0, -1e300, -1e300};
     for (int b = blockIdx.x * kBlock + threadIdx.x; b < nb; b += gridDim.x * kBlock) {
         const float fx = map_fix[b];
-        const float mv = (float)((double)acc_mov[b] * unit);
-        const float fd = rho_t * fx + (float)((double)acc_field[b] * unit);
+        const unsigned long long am = acc_mov[b], af = acc_field[b];
+        acc_mov[b] = 0ull;
+        acc_field[b] = 0ull;
+        const float mv = (float)((double)am * unit);
+        const float fd = rho_t * fx + (float)((double)af * unit);
         map_mov[b] = mv;
         map_field[b] = fd;
         const float v = mv + rho_t * fx - cap;
@@ -177,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_overflow(const unsigned long long *_
     block_reduce<4, 2>(acc, part + 4 * blockIdx.x);
 }
 
-// Map mean and centred range for the field solve: mean -> stats[0] (float),
+// Map mean and centred range for the field solve: mean -> mean_out (float),
 // max |map - mean| -> field_max0 (float bits).  One block.
 __global__ __launch_bounds__(kBlock) void k_map_stats(const double *__restrict__ part, int blocks,
                                                       int nb, float *__restrict__ mean_out,
@@ -373,12 +376,6 @@ __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
                                    DevScalars *__restrict__ sc, const CtlParams &p,
                                    ep_step_info *__restrict__ log, float *__restrict__ rec);
 
-__device__ inline double load_agent(const double *p)
-{
-    return __builtin_bit_cast(double, __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
-                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     const double *__restrict__ part, int wa_off, int wa_blocks, int ovf_off, int ovf_blocks,
     int cell_off, int cell_blocks, double mov_area, DevScalars *__restrict__ sc,
@@ -415,13 +412,18 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
         last = atomicAdd(ticket, 1u) == (unsigned)(B - 1);
     }
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
+    if (!last) return;
     __threadfence();
+    __shared__ double sfin[kFinalizeBlocks][10];  // every block's sums, loaded in parallel
+    if (threadIdx.x < B)
+        for (int k = 0; k < 10; ++k) sfin[threadIdx.x][k] = load_agent(&fin[threadIdx.x * 10 + k]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     double t[10];
-    for (int k = 0; k < 10; ++k) t[k] = load_agent(&fin[k]);
+    for (int k = 0; k < 10; ++k) t[k] = sfin[0][k];
     for (int j = 1; j < B; ++j) {
-        for (int k = 0; k < 9; ++k) t[k] += load_agent(&fin[j * 10 + k]);
-        t[9] = fmax(t[9], load_agent(&fin[j * 10 + 9]));
+        for (int k = 0; k < 9; ++k) t[k] += sfin[j][k];
+        t[9] = fmax(t[9], sfin[j][9]);
     }
     __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // next launch
     DevResult R;
@@ -533,24 +535,25 @@ __global__ void k_cell_bins(const float2 *__restrict__ pos, const float2 *__rest
 
 __device__ inline float lg10_clamped(double v) { return (float)sched_log10(fmax(v, 1e-30)); }
 
-// Step-model features of the coming iteration (order: include/eplace_mlp.h).
-__device__ inline void ctl_features(const DevCtl &S, double bw, float *f)
+// Step-model features of the coming iteration (order and definitions:
+// include/eplace_mlp.h); the host's step_features computes the same values.
+__device__ inline void ctl_features(const DevCtl &S, double bw, double isn, float *f)
 {
 #pragma clang fp contract(off)
-    f[0] = lg10_clamped(S.alpha);
-    f[1] = lg10_clamped(S.h_alpha1);
-    f[2] = lg10_clamped(S.h_alpha2);
+    f[0] = lg10_clamped(S.alpha / bw);
+    f[1] = lg10_clamped(S.h_alpha1 / bw);
+    f[2] = lg10_clamped(S.h_alpha2 / bw);
     f[3] = (float)S.last_tau;
     f[4] = (float)(S.last_tau - S.h_prev_tau);
-    f[5] = lg10_clamped(S.lambda);
+    f[5] = lg10_clamped(S.lambda / S.lambda0);
     f[6] = lg10_clamped(S.gamma / bw);
     f[7] = (float)((S.last_hpwl - S.h_prev_hpwl) / fmax(S.last_hpwl, 1e-30));
-    f[8] = lg10_clamped(S.h_gnorm);
-    f[9] = lg10_clamped(S.h_dv);
-    f[10] = lg10_clamped(S.h_dg);
+    f[8] = lg10_clamped(S.h_gnorm * isn);
+    f[9] = lg10_clamped(S.h_dv * isn / bw);
+    f[10] = lg10_clamped(S.h_dg * isn);
     f[11] = (float)S.coef;
     f[12] = (float)S.h_evals;
-    f[13] = lg10_clamped(S.h_sum_ratio);
+    f[13] = lg10_clamped(S.lambda / S.h_sum_ratio);
     f[14] = lg10_clamped(S.h_ratio);
     f[15] = (float)(S.iteration / 1000.0);
 }
@@ -561,19 +564,21 @@ __device__ inline void ctl_begin(DevCtl &S, DevScalars *sc, const CtlParams &p)
 #pragma clang fp contract(off)
     S.ak1 = 0.5 * (1.0 + sqrt(4.0 * S.ak * S.ak + 1.0));
     S.coef = (S.ak - 1.0) / S.ak1;
-    ctl_features(S, p.bw, S.feat);
+    ctl_features(S, p.bw, p.inv_sqrt_np, S.feat);
     for (int k = 0; k < 16; ++k) sc->feat[k] = S.feat[k];
     S.trials = 0;
     S.t = 0;
     S.learned_ok = 0;
-    S.learned_used = p.learned ? 1 : 0;
-    S.phase = p.learned ? 1 : 0;
-    if (p.learned) S.learned_proposals++;
+    const bool propose = p.learned && S.model_skip == 0;
+    if (p.learned && !propose) S.model_skip--;
+    S.learned_used = propose ? 1 : 0;
+    S.phase = propose ? 1 : 0;
+    if (propose) S.learned_proposals++;
     sc->alpha = (float)S.alpha;
     sc->coef = (float)S.coef;
     sc->gamma_inv = (float)(1.0 / S.gamma);
     sc->lambda = (float)S.lambda;
-    sc->use_model = p.learned ? 1 : 0;
+    sc->use_model = propose ? 1 : 0;
 }
 
 // The decision after a trial whose reductions are R (one thread).
@@ -602,7 +607,10 @@ __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
             accept = true;
             S.learned_ok = 1;
             S.learned_accepted++;
-        } else {  // fall back to the search from the Lipschitz step
+            S.model_backoff = 0;
+        } else {  // fall back to the search from the Lipschitz step; the model sits out
+            S.model_backoff = S.model_backoff ? min(2 * S.model_backoff, kModelBackoffMax) : 2;
+            S.model_skip = S.model_backoff;
             S.phase = 0;
             S.t = 0;
             sc->use_model = 0;
@@ -618,7 +626,7 @@ __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
         if (S.n_rec < p.rec_cap) {
             float *r = rec + (size_t)S.n_rec * 17;
             for (int k = 0; k < 16; ++k) r[k] = S.feat[k];
-            r[16] = (float)sched_log10(fmax(used, 1e-30));
+            r[16] = (float)sched_log10(fmax(used / p.bw, 1e-30));
             S.n_rec++;
         }
         S.cur ^= 1;

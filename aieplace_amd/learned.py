@@ -1,10 +1,10 @@
 """Learned Nesterov step length: record collection, GPU training, weights I/O.
 
 The model (16 -> 64 -> 64 -> 1, GELU; include/eplace_mlp.h) is trained by the
-library's own HIP kernels (ep_mlp_train) on (features, accepted log10 step)
-records that ep_step collects during plain placements.  In a learned run each
-proposal is verified by the backtracking condition and falls back to the
-Lipschitz search when it fails.
+library's own HIP kernels (ep_mlp_train) on records (features, log10(accepted
+step / bin width)) that ep_step collects during plain placements.  In a
+learned run each proposal is verified by the backtracking condition and falls
+back to the Lipschitz search when it fails.
 
 Usage (on a GPU):  python -m aieplace_amd.learned --runs 200 --out <file.npy>
 """

@@ -35,7 +35,7 @@ FAMILIES = [
     ("k_sort_", "cell_sort"), ("k_density", "density_scatter"), ("k_overflow", "overflow"),
     ("k_map_stats", "overflow"), ("k_wa", "wa_grad"), ("k_sym_", "field_transforms"),
     ("k_clear_max", "field_transforms"), ("k_cell_grad", "cell_grad"), ("k_trial", "nesterov_trial"),
-    ("k_finalize", "finalize"), ("k_fill_maps", "fill_maps"),
+    ("k_finalize", "finalize"),
 ]
 
 

@@ -8,14 +8,19 @@
  *   [0,16)   feature mean      [16,32)  feature std
  *   32       target mean       33       target std
  *   W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3
- * The model sees x' = (x - mean)/std and predicts t' ; log10(step) = t' * std_t + mean_t.
+ * The model sees x' = (x - mean)/std and predicts t' ; log10(step / bin width) =
+ * t' * std_t + mean_t, so the proposed step is bin width * 10^(t' std_t + mean_t).
  * Loss: mean squared error of t' against the normalised target.
  *
- * Features (per iteration, in this order): log10 of the Lipschitz step, of the
- * two previous accepted steps; overflow; overflow change; log10 lambda;
- * log10(gamma / bin width); relative HPWL change; log10 |g|; log10 |dv|;
- * log10 |dg|; Nesterov coefficient; evaluations of the last iteration;
- * log10(|grad W|_1 / |grad D|_1); log10(last alpha_hat / alpha); iteration/1000.
+ * Features (per iteration, in this order; lengths in bin widths bw, norms per
+ * cell: |.| / sqrt(n), n = movable + filler cells, so that instances of any
+ * size and grid share one scale): log10(alpha_Lipschitz / bw), log10 of the
+ * two previous accepted steps / bw; overflow; overflow change;
+ * log10(lambda / lambda_0); log10(gamma / bw); relative HPWL change;
+ * log10(|g| / sqrt n); log10(|dv| / (sqrt n bw)); log10(|dg| / sqrt n);
+ * Nesterov coefficient; evaluations of the last iteration;
+ * log10(lambda |grad D|_1 / |grad W|_1); log10(last alpha_hat / alpha);
+ * iteration/1000.
  */
 #ifndef EPLACE_MLP_H
 #define EPLACE_MLP_H
@@ -46,7 +51,7 @@ void ep_mlp_default_train_params(ep_mlp_train_params *p);
  * training loss of each epoch. */
 ep_status ep_mlp_train(const float *features, const float *log_step, int32_t n,
                        const ep_mlp_train_params *p, float *weights, float *loss_curve);
-/* Inference for n feature rows: out = predicted log10 step. */
+/* Inference for n feature rows: out = predicted log10(step / bin width). */
 ep_status ep_mlp_forward(const float *weights, const float *features, int32_t n, float *out);
 /* Loss and its gradient w.r.t. the weight blob (normalisation entries get 0). */
 ep_status ep_mlp_grad(const float *weights, const float *features, const float *log_step, int32_t n,
