@@ -38,7 +38,7 @@ constexpr int kCellGridCap = 16384;
 // Spatial counting sort of cells: kSortSide x kSortSide coarse tiles.
 constexpr int kSortSide = 64;
 constexpr int kSortTiles = kSortSide * kSortSide;
-constexpr int kSortCellsPerThread = 8;
+constexpr int kSortCellsPerThread = 16;
 // After a rejected model proposal the model sits out 2, 4, ... up to this many
 // iterations (reset by an accepted proposal): phases where it mispredicts cost
 // one extra evaluation per rejection, so they are skipped.
