@@ -300,3 +300,32 @@ def test_device_control_matches_host_control(precision, learned_step):
         assert a == b
     assert np.array_equal(x0, x1) and np.array_equal(y0, y1)
     assert np.array_equal(f0, f1) and np.array_equal(t0, t1)
+
+
+def test_device_run_hands_state_to_host_steps():
+    """A device-controlled run that stops at max_iters leaves the optimiser state
+    (step lengths, lambda, gamma, histories, the accepted buffers) with the
+    host: continuing with ep_step gives exactly the trajectory of a run that
+    used ep_step from the start.  The log holds at most log_cap entries."""
+    say("case device run -> host steps hand-off")
+    nl, p = make("stdcell", 20_000, 128, use_graph=1, max_iters=40)
+    with A.Placer(nl, p) as pl:
+        r = pl.run(log_cap=2000)
+        assert r["iterations"] == 40 and len(r["log"]) == 40
+        tail = [pl.nudgeAllNodes() for _ in range(20)]
+        xa, ya = pl.get_positions()
+    p.use_graph = 0
+    with A.Placer(nl, p) as pl:
+        pl.initializePlacement()
+        ref = [pl.nudgeAllNodes() for _ in range(60)]
+        xb, yb = pl.get_positions()
+    keys = ("iteration", "evals", "hpwl", "overflow", "lam", "gamma", "alpha")
+    for a, b in zip(r["log"], ref[:40]):
+        assert all(a[k] == b[k] for k in keys), (a, b)
+    for a, b in zip(tail, ref[40:]):
+        assert all(a[k] == b[k] for k in keys), (a, b)
+    assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+    p.use_graph = 1
+    with A.Placer(nl, p) as pl:
+        short = pl.run(log_cap=7)
+        assert short["iterations"] == 40 and len(short["log"]) == 7
