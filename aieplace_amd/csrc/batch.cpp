@@ -88,6 +88,12 @@ struct ep_batch {
     int n_buckets = 0;
     double *part_wa = nullptr, *part_ovf = nullptr, *part_cell = nullptr;
     int cell_blocks = 0, sort_blocks = 0, bin_blocks = 0, wa_blocks = 0, dens_blocks[3] = {};
+    // Per-instance block counts and WA buckets (blk0 relative to the instance):
+    // the block tables cover the instances in `active` only and are rebuilt
+    // from these when finished instances are dropped (set_tables).
+    std::vector<int32_t> nb_cell, nb_sort, nb_bin, nb_wa, nb_dens[3];
+    std::vector<std::vector<BBucket>> inst_buckets;
+    std::vector<char> active;
     int dens_variant = 1;
     BatchField *bf = nullptr;
     BState *h_st = nullptr;  // pinned
@@ -178,8 +184,58 @@ static void destroy_batch(ep_batch *b)
 }
 
 // Initial placement and a fresh optimiser state on the device.
+// Block tables, WA buckets and transform tile lists for the instances with
+// active[i]; the others get no blocks (their kernels' blocks and tiles vanish
+// from the launches instead of exiting one by one).  Every instance keeps its
+// storage and block-local work, so results do not change.
+static ep_status set_tables(ep_batch *b, const std::vector<char> &active)
+{
+    const int n = b->n;
+    std::vector<int32_t> pc{0}, ps{0}, pb{0}, pw{0}, pd[3];
+    for (int v = 0; v < 3; ++v) pd[v].push_back(0);
+    std::vector<BBucket> bks;
+    for (int i = 0; i < n; ++i) {
+        const bool a = active[i] != 0;
+        if (a)
+            for (const BBucket &bk : b->inst_buckets[i]) {
+                BBucket x = bk;
+                x.blk0 += pw.back();
+                bks.push_back(x);
+            }
+        pc.push_back(pc.back() + (a ? b->nb_cell[i] : 0));
+        ps.push_back(ps.back() + (a ? b->nb_sort[i] : 0));
+        pb.push_back(pb.back() + (a ? b->nb_bin[i] : 0));
+        pw.push_back(pw.back() + (a ? b->nb_wa[i] : 0));
+        for (int v = 0; v < 3; ++v) pd[v].push_back(pd[v].back() + (a ? b->nb_dens[v][i] : 0));
+    }
+    EPB_HIP(hipMemcpy(b->pre_cell, pc.data(), pc.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    EPB_HIP(hipMemcpy(b->pre_sort, ps.data(), ps.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    EPB_HIP(hipMemcpy(b->pre_bin, pb.data(), pb.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    EPB_HIP(hipMemcpy(b->pre_wa, pw.data(), pw.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    for (int v = 0; v < 3; ++v)
+        EPB_HIP(hipMemcpy(b->pre_dens[v], pd[v].data(), pd[v].size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!bks.empty())
+        EPB_HIP(hipMemcpy(b->buckets, bks.data(), bks.size() * sizeof(BBucket), hipMemcpyHostToDevice));
+    b->n_buckets = (int)bks.size();
+    b->cell_blocks = pc.back();
+    b->sort_blocks = ps.back();
+    b->bin_blocks = pb.back();
+    b->wa_blocks = pw.back();
+    for (int v = 0; v < 3; ++v) b->dens_blocks[v] = pd[v].back();
+    ep_status st = batch_field_set_active(b->bf, active);
+    if (st != EP_OK) return st;
+    b->active = active;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);  // captured with the old grids
+    b->exec = nullptr;
+    return EP_OK;
+}
+
 static ep_status reset_state(ep_batch *b)
 {
+    if (std::find(b->active.begin(), b->active.end(), 0) != b->active.end()) {
+        ep_status st = set_tables(b, std::vector<char>(b->n, 1));
+        if (st != EP_OK) return st;
+    }
     const size_t nc = (size_t)b->total_cells;
     for (int k = 0; k < 2; ++k) {
         EPB_HIP(hipMemcpy(b->V[k], b->init_pos.data(), nc * sizeof(float2), hipMemcpyHostToDevice));
@@ -322,10 +378,14 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
             pin_cell.push_back((int32_t)(cell_off + L.pin_cell[q]));
             pin_off.push_back(L.pin_off[q]);
         }
+        b->inst_buckets.emplace_back();
         for (int k = 0; k < L.buckets.n; ++k) {
             const Bucket &bk = L.buckets.b[k];
             bks.push_back({i, bk.lanes, (int32_t)(net_base + bk.net0), (int32_t)(net_base + bk.net1),
                            (int32_t)(wa_base + bk.blk0), bk.deg, (int32_t)(pin_base + bk.pin0)});
+            b->inst_buckets.back().push_back({i, bk.lanes, (int32_t)(net_base + bk.net0),
+                                              (int32_t)(net_base + bk.net1), bk.blk0, bk.deg,
+                                              (int32_t)(pin_base + bk.pin0)});
         }
         // cell -> pin CSR over all cells (empty for fillers, fixed and padding)
         for (int64_t k = 0; k < span; ++k)
@@ -345,6 +405,12 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
         pre_bin.push_back(pre_bin.back() + (int32_t)(((int64_t)bins[i] * bins[i] + kBinsPerBlock - 1) / kBinsPerBlock));
         wa_base += L.wa_blocks;
         pre_wa.push_back((int32_t)wa_base);
+        const size_t ni = pre_cell.size() - 1;
+        b->nb_cell.push_back(pre_cell[ni] - pre_cell[ni - 1]);
+        b->nb_sort.push_back(pre_sort[ni] - pre_sort[ni - 1]);
+        b->nb_bin.push_back(pre_bin[ni] - pre_bin[ni - 1]);
+        b->nb_wa.push_back(pre_wa[ni] - pre_wa[ni - 1]);
+        for (int v = 0; v < 3; ++v) b->nb_dens[v].push_back(pre_dens[v][ni] - pre_dens[v][ni - 1]);
         cell_off += span;
         map_off += (int64_t)bins[i] * bins[i];
         pin_base += L.n_pins;
@@ -353,6 +419,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     }
     cps.push_back((int32_t)cpin_base);
     b->total_cells = cell_off;
+    b->active.assign(n, 1);
     b->init_pos = pos;
     b->n_buckets = (int)bks.size();
     b->cell_blocks = pre_cell.back();
@@ -555,11 +622,19 @@ extern "C" ep_status ep_batch_run(ep_batch *b, int32_t max_launches, ep_batch_ru
             if (!b->h_st[i].done) return false;
         return true;
     };
+    auto count = [](const std::vector<char> &a) { return (int)std::count(a.begin(), a.end(), 1); };
     while (launches < max_launches && !all_done()) {
         EPB_HIP(hipGraphLaunch(b->exec, b->stream));
         EPB_HIP(hipStreamSynchronize(b->stream));
         ++launches;
         if ((st = check_errors(b)) != EP_OK) return st;
+        // once half of the instances in the launches have finished, drop them
+        std::vector<char> running(b->n, 0);
+        for (int i = 0; i < b->n; ++i) running[i] = b->active[i] && !b->h_st[i].done;
+        const int r = count(running);
+        if (r > 0 && 2 * r <= count(b->active)) {
+            if ((st = set_tables(b, running)) != EP_OK || (st = build_graph(b)) != EP_OK) return st;
+        }
     }
     const auto t1 = std::chrono::steady_clock::now();
     if (info) {

@@ -133,6 +133,8 @@ struct BatchFieldInst {
 };
 struct BatchField;
 ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &inst);
+// Tile lists of the instances with active[i] only (finished instances get no tiles).
+ep_status batch_field_set_active(BatchField *bf, const std::vector<char> &active);
 void batch_field_free(BatchField *bf);
 void launch_field_batch(BatchField *bf, int precision, hipStream_t s);
 

@@ -909,6 +909,7 @@ struct BatchField {
     int n_tiles[4] = {};
     uint32_t *fmax = nullptr;                // [4 * n_inst]
     int n_inst = 0;
+    std::vector<int> M_of;                   // grid side of each instance
 };
 
 ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &inst)
@@ -917,6 +918,7 @@ ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &
     if (!bf) return fail(EP_ENOMEM, "batch field: host allocation failed");
     out = bf;
     bf->n_inst = (int)inst.size();
+    for (const BatchFieldInst &d : inst) bf->M_of.push_back(d.M);
     for (const BatchFieldInst &d : inst)
         if (std::find(bf->sizes.begin(), bf->sizes.end(), d.M) == bf->sizes.end()) bf->sizes.push_back(d.M);
     bf->sets.resize(bf->sizes.size());
@@ -993,6 +995,28 @@ ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &
     return EP_OK;
 }
 
+ep_status batch_field_set_active(BatchField *bf, const std::vector<char> &active)
+{
+    std::vector<int4> tiles[4];
+    for (int i = 0; i < bf->n_inst; ++i) {
+        if (!active[i]) continue;
+        const int M = bf->M_of[i];
+        const int mt = ((M >> 1) + SBM - 1) / SBM, nt = (M + SBN - 1) / SBN;
+        for (int stage = 0; stage < 4; ++stage)
+            for (int z = 0; z < (stage >= 2 ? 2 : 1); ++z)
+                for (int r = 0; r < mt; ++r)
+                    for (int q = 0; q < nt; ++q) tiles[stage].push_back(make_int4(i, r * SBM, q * SBN, z));
+    }
+    for (int stage = 0; stage < 4; ++stage) {  // the lists only shrink: the buffers fit
+        bf->n_tiles[stage] = (int)tiles[stage].size();
+        if (!tiles[stage].empty() &&
+            hipMemcpy(bf->d_tiles[stage], tiles[stage].data(), sizeof(int4) * tiles[stage].size(),
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return fail(EP_EDEVICE, "batch field: tile upload failed");
+    }
+    return EP_OK;
+}
+
 void batch_field_free(BatchField *bf)
 {
     if (!bf) return;
@@ -1013,6 +1037,7 @@ void launch_field_batch(BatchField *bf, int precision, hipStream_t s)
     if (precision == EP_PREC_FP16X3)
         hipLaunchKernelGGL(k_clear_max_b, dim3((bf->n_inst + 63) / 64), dim3(64), 0, s, bf->fmax, bf->n_inst);
     for (int stage = 0; stage < 4; ++stage) {
+        if (bf->n_tiles[stage] == 0) continue;
         const dim3 grid(bf->n_tiles[stage]), blk(256);
         const GemmArgs *args = bf->d_args[pi][stage];
         const int4 *tiles = bf->d_tiles[stage];
