@@ -249,6 +249,9 @@ static void destroy_ctx(ep_ctx *c)
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->h_res) (void)hipHostFree(c->h_res);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -519,9 +522,12 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
 
     // ---- device buffers
     hipError_t he = hipStreamCreate(&c->stream);
+    if (he == hipSuccess) he = hipStreamCreate(&c->side);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
     if (he != hipSuccess) {
         destroy_ctx(c);
-        return fail(EP_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(he));
+        return fail(EP_EDEVICE, "stream/event creation: %s", hipGetErrorString(he));
     }
     const size_t nb = (size_t)c->Mx * c->My;
     c->ovf_blocks = (int)std::min<size_t>(1024, (nb + kBlock * 4 - 1) / (kBlock * 4));
@@ -1142,8 +1148,14 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
         launch_sort(x, vt, true, s);
         launch_density(x, s);
         launch_overflow(x, s);
-        launch_wa(x, vt, s);
+        // fork: the WA gradient beside the transform products, which leave
+        // most of each CU's wave slots free
+        EP_HIP(hipEventRecord(x.fork, s));
+        EP_HIP(hipStreamWaitEvent(x.side, x.fork, 0));
+        launch_wa(x, vt, x.side);
+        EP_HIP(hipEventRecord(x.join, x.side));
         launch_field(x, x.prm.precision, s);
+        EP_HIP(hipStreamWaitEvent(s, x.join, 0));
         launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s);
         launch_finalize_decide(x, p, s);
     }

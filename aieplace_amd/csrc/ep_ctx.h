@@ -225,6 +225,10 @@ void field_free(Ctx &c);
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Second branch of the captured evaluation: the WA gradient runs beside the
+    // sort -> density -> field chain (it needs only the trial positions).
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 
     // sizes
     int32_t n_mov = 0, n_fill = 0, n_fix = 0, n_all = 0, n_place = 0;
