@@ -242,7 +242,7 @@ static void destroy_ctx(ep_ctx *c)
                     c->map_field, c->map_mov, c->map_fix, c->acc_field, c->acc_mov, c->exy, c->tA, c->tB, c->part,
                     c->sorted, c->sort_count, c->sort_off, c->dens_tab,
                     c->d_scal, c->d_res, c->d_mlp_w, c->d_ctl, c->d_log, c->d_rec, c->fin,
-                    c->fin_ticket};
+                    c->fin_ticket, c->gw_cell};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     field_free(*c);
@@ -550,6 +550,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     EP_TRY(dalloc(&c->pin_off, L.n_pins));
     EP_TRY(dalloc(&c->pin_grad, L.n_pins));
     EP_TRY(dalloc(&c->cell_pin_start, c->n_mov + 1));
+    EP_TRY(dalloc(&c->gw_cell, std::max(1, c->n_mov)));
     EP_TRY(dalloc(&c->cell_pins, L.cpins.size()));
     for (int i = 0; i < 2; ++i) {
         EP_TRY(dalloc(&c->V[i], c->n_all));
@@ -1153,10 +1154,11 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
         EP_HIP(hipEventRecord(x.fork, s));
         EP_HIP(hipStreamWaitEvent(x.side, x.fork, 0));
         launch_wa(x, vt, x.side);
+        launch_pin_sum(x, x.side);
         EP_HIP(hipEventRecord(x.join, x.side));
         launch_field(x, x.prm.precision, s);
         EP_HIP(hipStreamWaitEvent(s, x.join, 0));
-        launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s);
+        launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s, x.gw_cell);
         launch_finalize_decide(x, p, s);
     }
     EP_HIP(hipStreamEndCapture(s, &g));

@@ -184,9 +184,13 @@ void launch_density(Ctx &c, hipStream_t s);
 void launch_overflow(Ctx &c, hipStream_t s);
 void launch_wa(Ctx &c, Pair<const float2> pos, hipStream_t s);
 void launch_field(Ctx &c, int precision, hipStream_t s);
+// gw_cell (nullable): per-cell pin-partial sums made by launch_pin_sum; without
+// it the kernel gathers them itself.
 void launch_cell_grad(Ctx &c, Pair<const float2> pos, Pair<const float2> vref,
                       Pair<const float2> gref, Pair<float2> gout, float2 *gw_out, float2 *f_out,
-                      hipStream_t s);
+                      hipStream_t s, const float2 *gw_cell = nullptr);
+// gw_cell[c] = sum of movable cell c's pin partials (after launch_wa).
+void launch_pin_sum(Ctx &c, hipStream_t s);
 void launch_finalize(Ctx &c, hipStream_t s);
 // Nesterov trial into v_new/u_new; `count`: also count v_new into the sort's
 // tiles (the following evaluation then sorts with counted = true).
@@ -243,6 +247,7 @@ struct Ctx {
     float2 *wh = nullptr;
     int32_t *net_start = nullptr, *pin_cell = nullptr;
     float2 *pin_off = nullptr, *pin_grad = nullptr;
+    float2 *gw_cell = nullptr;  // per movable cell: sum of its pin partials
     int32_t *cell_pin_start = nullptr, *cell_pins = nullptr;
     BucketTable buckets{};
     int32_t wa_blocks = 0;

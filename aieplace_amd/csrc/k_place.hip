@@ -311,7 +311,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
     const int32_t *__restrict__ cell_pins, const float2 *__restrict__ pin_grad,
     const float2 *__restrict__ exy, Pair<const float2> vref_pair,
     Pair<const float2> gref_pair, Pair<float2> gout_pair, float2 *__restrict__ gw_out,
-    float2 *__restrict__ f_out, const DevScalars *__restrict__ sc, double *__restrict__ part)
+    float2 *__restrict__ f_out, const DevScalars *__restrict__ sc, double *__restrict__ part,
+    const float2 *__restrict__ gw_cell)
 {
     if (sc->halt) return;
     const int sel = sc->sel;
@@ -334,7 +335,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
         }
         const int np = k1 - k0;
         const float2 f = cell_force(p, s, exy, g, Mx, My);
-        const float2 gw = sum_pin_partials(k0, k1, cell_pins, pin_grad);
+        const float2 gw = gw_cell ? (c < n_mov ? gw_cell[c] : make_float2(0.f, 0.f))
+                                  : sum_pin_partials(k0, k1, cell_pins, pin_grad);
         const float gwx = gw.x, gwy = gw.y, fx = f.x, fy = f.y;
         const float pre = fmaxf(1.0f, (float)np + lambda * s.x * s.y);
         const float gx = (gwx - lambda * fx) / pre, gy = (gwy - lambda * fy) / pre;
@@ -356,6 +358,21 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
         acc[6] = fmax(acc[6], fmax(fabs((double)gx), fabs((double)gy)));
     }
     block_reduce<7, 1>(acc, part + 7 * blockIdx.x);
+}
+
+// Per movable cell, the sum of its pin partials (what k_cell_grad gathers
+// itself when it gets no gw_cell): lets the gather run beside the transform
+// products, off the path to the field.
+__global__ __launch_bounds__(kBlock) void k_pin_sum(const int32_t *__restrict__ cell_pin_start,
+                                                   const int32_t *__restrict__ cell_pins,
+                                                   const float2 *__restrict__ pin_grad, int n_mov,
+                                                   float2 *__restrict__ gw_cell,
+                                                   const DevScalars *__restrict__ sc)
+{
+    if (sc->halt) return;
+    for (int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; c < n_mov;
+         c += gridDim.x * kBlock)
+        gw_cell[c] = sum_pin_partials(cell_pin_start[c], cell_pin_start[c + 1], cell_pins, pin_grad);
 }
 
 // Deterministic final reduction of all partial sums: block b of
@@ -764,14 +781,21 @@ void launch_wa(Ctx &c, Pair<const float2> pos, hipStream_t s)
                        c.part + c.part_wa_off);
 }
 
+void launch_pin_sum(Ctx &c, hipStream_t s)
+{
+    if (c.n_mov == 0) return;
+    hipLaunchKernelGGL(k_pin_sum, dim3(grid_for(c.n_mov, kBlock, kCellGridCap)), dim3(kBlock), 0, s,
+                       c.cell_pin_start, c.cell_pins, c.pin_grad, c.n_mov, c.gw_cell, c.d_scal);
+}
+
 void launch_cell_grad(Ctx &c, Pair<const float2> pos, Pair<const float2> vref,
                       Pair<const float2> gref, Pair<float2> gout, float2 *gw_out, float2 *f_out,
-                      hipStream_t s)
+                      hipStream_t s, const float2 *gw_cell)
 {
     hipLaunchKernelGGL(k_cell_grad, dim3(c.cell_blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
                        c.n_mov, geom_args(c), c.Mx, c.My, c.cell_pin_start, c.cell_pins,
                        c.pin_grad, c.exy, vref, gref, gout, gw_out, f_out, c.d_scal,
-                       c.part + c.part_cell_off);
+                       c.part + c.part_cell_off, gw_cell);
 }
 
 void launch_finalize(Ctx &c, hipStream_t s)
