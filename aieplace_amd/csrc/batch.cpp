@@ -61,6 +61,8 @@ ep_status upload(T *dst, const std::vector<T> &src)
 struct ep_batch {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;              // second captured branch (WA beside the maps and field)
+    hipEvent_t fork = nullptr, join = nullptr;
     ep_params prm{};
     int n = 0;
     std::vector<BInst> inst;
@@ -152,6 +154,25 @@ struct ep_batch {
         return d;
     }
 
+    // One gradient evaluation as a captured fork: the WA gradient on the side
+    // stream beside the density scatter and the transform products.
+    hipError_t eval_forked(const BDev &d, int at, bool counted, hipStream_t s)
+    {
+        const int v = dens_variant;
+        bl_sort(d, at, counted, bin_blocks, sort_blocks, kDensCpt[v], s);
+        hipError_t e = hipEventRecord(fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+        if (e != hipSuccess) return e;
+        bl_wa(d, at, wa_blocks, side);
+        if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
+        bl_dens(d, bin_blocks, dens_blocks[v], kDensCpt[v], s);
+        launch_field_batch(bf, prm.precision, s);
+        if ((e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
+        bl_cell_grad(d, at, cell_blocks, s);
+        bl_reduce(d, s);
+        return hipSuccess;
+    }
+
     // One gradient evaluation of every unfinished instance (`counted`: the
     // preceding trial move counted the sort tiles).
     void eval(const BDev &d, int at, bool counted, hipStream_t s)
@@ -179,6 +200,9 @@ static void destroy_batch(ep_batch *b)
         if (p) (void)hipFree(p);
     batch_field_free(b->bf);
     if (b->h_st) (void)hipHostFree(b->h_st);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -443,9 +467,11 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
         }                         \
     } while (0)
     (void)hipGetDevice(&b->device);
-    if (hipStreamCreate(&b->stream) != hipSuccess) {
+    if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->side) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
         destroy_batch(b);
-        return fail(EP_EDEVICE, "ep_batch_create: hipStreamCreate failed");
+        return fail(EP_EDEVICE, "ep_batch_create: stream/event creation failed");
     }
     const size_t nc = (size_t)cell_off, nb = (size_t)map_off;
     EPB_TRY(dalloc(&b->d_inst, n));
@@ -597,7 +623,7 @@ static ep_status build_graph(ep_batch *b)
     EPB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int t = 0; t < kTrialsPerLaunch; ++t) {
         bl_move(d, kMoveTrial, b->sort_blocks, b->stream);
-        b->eval(d, kAtTrial, true, b->stream);
+        EPB_HIP(b->eval_forked(d, kAtTrial, true, b->stream));
         bl_control(d, kCtlIterate, b->stream);
     }
     (void)hipMemcpyAsync(b->h_st, b->d_st, sizeof(BState) * b->n, hipMemcpyDeviceToHost, b->stream);

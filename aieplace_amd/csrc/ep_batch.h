@@ -112,6 +112,10 @@ enum BMoveMode { kMoveTrial = 0, kMoveProbe = 1, kMoveInitU = 2 };
 void bl_move(const BDev &d, int mode, int sort_blocks, hipStream_t s);
 // Density maps of every unfinished instance; `counted`: a kMoveTrial move made
 // the sort counts at the evaluated positions.
+// bl_maps = bl_sort (clear, tile sort) then bl_dens (scatter, overflow, map statistics).
+void bl_sort(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_cpt,
+             hipStream_t s);
+void bl_dens(const BDev &d, int bin_blocks, int dens_blocks, int dens_cpt, hipStream_t s);
 void bl_maps(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_blocks,
              int dens_cpt, hipStream_t s);
 void bl_wa(const BDev &d, int at, int wa_blocks, hipStream_t s);

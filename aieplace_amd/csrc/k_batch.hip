@@ -542,13 +542,17 @@ void bl_move(const BDev &d, int mode, int sort_blocks, hipStream_t s)
     hipLaunchKernelGGL(kb_move, dim3(sort_blocks), dim3(kBlock), 0, s, d, mode);
 }
 
-void bl_maps(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_blocks,
-             int dens_cpt, hipStream_t s)
+void bl_sort(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_cpt,
+             hipStream_t s)
 {
     hipLaunchKernelGGL(kb_fill, dim3(bin_blocks), dim3(kBlock), 0, s, d);
     if (!counted) hipLaunchKernelGGL(kb_sort_count, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
     hipLaunchKernelGGL(kb_sort_scan, dim3(d.n_inst), dim3(kSortScanThreads), 0, s, d, kBlock * dens_cpt);
     hipLaunchKernelGGL(kb_sort_scatter, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
+}
+
+void bl_dens(const BDev &d, int bin_blocks, int dens_blocks, int dens_cpt, hipStream_t s)
+{
     const size_t lds = 2 * sizeof(unsigned long long) * (size_t)d.dens_wcap;
     if (dens_cpt == 2)
         hipLaunchKernelGGL(kb_density<2>, dim3(dens_blocks), dim3(kBlock), lds, s, d);
@@ -558,6 +562,13 @@ void bl_maps(const BDev &d, int at, bool counted, int bin_blocks, int sort_block
         hipLaunchKernelGGL(kb_density<4>, dim3(dens_blocks), dim3(kBlock), lds, s, d);
     hipLaunchKernelGGL(kb_overflow, dim3(bin_blocks), dim3(kBlock), 0, s, d);
     hipLaunchKernelGGL(kb_map_stats, dim3(d.n_inst), dim3(kBlock), 0, s, d);
+}
+
+void bl_maps(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_blocks,
+             int dens_cpt, hipStream_t s)
+{
+    bl_sort(d, at, counted, bin_blocks, sort_blocks, dens_cpt, s);
+    bl_dens(d, bin_blocks, dens_blocks, dens_cpt, s);
 }
 
 void bl_wa(const BDev &d, int at, int wa_blocks, hipStream_t s)
