@@ -154,6 +154,14 @@ struct ep_batch {
         return d;
     }
 
+    int64_t active_cells() const
+    {
+        int64_t n_cells = 0;
+        for (int i = 0; i < n; ++i)
+            if (active[i]) n_cells += inst[i].n_place;
+        return n_cells;
+    }
+
     // One gradient evaluation as a captured fork: the WA gradient on the side
     // stream beside the density scatter and the transform products.
     hipError_t eval_forked(const BDev &d, int at, bool counted, hipStream_t s)
@@ -623,7 +631,8 @@ static ep_status build_graph(ep_batch *b)
     EPB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int t = 0; t < kTrialsPerLaunch; ++t) {
         bl_move(d, kMoveTrial, b->sort_blocks, b->stream);
-        EPB_HIP(b->eval_forked(d, kAtTrial, true, b->stream));
+        if (b->active_cells() >= kForkMinCells) EPB_HIP(b->eval_forked(d, kAtTrial, true, b->stream));
+        else b->eval(d, kAtTrial, true, b->stream);
         bl_control(d, kCtlIterate, b->stream);
     }
     (void)hipMemcpyAsync(b->h_st, b->d_st, sizeof(BState) * b->n, hipMemcpyDeviceToHost, b->stream);

@@ -1147,18 +1147,26 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
         launch_mlp_propose(x, s);
         launch_trial(x, vc, uc, gc, vn, un, true, s);
         launch_sort(x, vt, true, s);
-        // fork: the WA gradient and the per-cell pin sums beside the density
-        // scatter and the transform products
-        EP_HIP(hipEventRecord(x.fork, s));
-        EP_HIP(hipStreamWaitEvent(x.side, x.fork, 0));
-        launch_wa(x, vt, x.side);
-        launch_pin_sum(x, x.side);
-        EP_HIP(hipEventRecord(x.join, x.side));
-        launch_density(x, s);
-        launch_overflow(x, s);
-        launch_field(x, x.prm.precision, s);
-        EP_HIP(hipStreamWaitEvent(s, x.join, 0));
-        launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s, x.gw_cell);
+        if (x.n_place >= kForkMinCells) {
+            // fork: the WA gradient and the per-cell pin sums beside the density
+            // scatter and the transform products
+            EP_HIP(hipEventRecord(x.fork, s));
+            EP_HIP(hipStreamWaitEvent(x.side, x.fork, 0));
+            launch_wa(x, vt, x.side);
+            launch_pin_sum(x, x.side);
+            EP_HIP(hipEventRecord(x.join, x.side));
+            launch_density(x, s);
+            launch_overflow(x, s);
+            launch_field(x, x.prm.precision, s);
+            EP_HIP(hipStreamWaitEvent(s, x.join, 0));
+            launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s, x.gw_cell);
+        } else {
+            launch_density(x, s);
+            launch_overflow(x, s);
+            launch_wa(x, vt, s);
+            launch_field(x, x.prm.precision, s);
+            launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s);
+        }
         launch_finalize_decide(x, p, s);
     }
     EP_HIP(hipStreamEndCapture(s, &g));

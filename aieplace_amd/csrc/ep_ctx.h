@@ -43,6 +43,10 @@ constexpr int kSortCellsPerThread = 16;
 // iterations (reset by an accepted proposal): phases where it mispredicts cost
 // one extra evaluation per rejection, so they are skipped.
 constexpr int kModelBackoffMax = 32;
+// Captured evaluations fork the WA gradient onto a second branch only from this
+// many placeable cells on: the fork and join cost ~15 us per trial, more than the
+// overlap saves on small instances.
+constexpr int64_t kForkMinCells = 400000;
 // Step-model records kept per context between ep_take_step_records calls.
 constexpr size_t kMaxStepRecords = (size_t)1 << 21;
 // Default HPWL reference of the lambda schedule: this many bin widths per net.
