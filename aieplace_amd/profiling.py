@@ -125,7 +125,10 @@ def stats(args) -> str:
              "Command (from the repository root):", "", "```", " ".join(full[:]).replace(sys.executable, "python3"),
              "```", "",
              f"One full placement (graph replays) followed by {args.reps} ep_profile repetitions of each "
-             "kernel at the final placement.", ""]
+             "kernel at the final placement.  In the placement the WA gradient and the per-cell pin sums "
+             "(k_wa, k_pin_sum) run on a second captured branch beside the density scatter and the "
+             "transform products, so those kernels' durations include the sharing of the CUs and the "
+             "shares add up to more than the wall time; ep_profile (below) times each item alone.", ""]
     if res:
         r = res["run"]
         lines += [f"Run: {r['iterations']} iterations, {r['evals']} gradient evaluations, "
