@@ -107,7 +107,8 @@ class EpBatchRunInfo(C.Structure):
     _fields_ = [("instances", C.c_int32), ("finished", C.c_int32),
                 ("instance_iterations", C.c_int64), ("evals", C.c_int64),
                 ("launches", C.c_int32), ("seconds", C.c_double),
-                ("density_cells_per_thread", C.c_int32)]
+                ("density_cells_per_thread", C.c_int32),
+                ("transform_k_step", C.c_int32)]
 
 
 class EplaceError(RuntimeError):

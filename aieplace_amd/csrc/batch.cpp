@@ -299,7 +299,7 @@ static ep_status pull_state(ep_batch *b)
 }
 
 // Picks the density-scatter tile (cells per thread) by timing each variant on
-// the current placement.
+// the current placement, then the transform products' K step.
 static ep_status autotune(ep_batch *b)
 {
     hipEvent_t e0, e1;
@@ -332,7 +332,8 @@ static ep_status autotune(ep_batch *b)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->dens_variant = best_v;
-    return EP_OK;
+    // the maps now hold the initial placement's density: tune the transform K step
+    return batch_field_autotune(b->bf, kTuneReps, b->stream);
 }
 
 extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t *bins, int32_t n,
@@ -685,6 +686,7 @@ extern "C" ep_status ep_batch_run(ep_batch *b, int32_t max_launches, ep_batch_ru
         info->launches = launches;
         info->seconds = std::chrono::duration<double>(t1 - t0).count();
         info->density_cells_per_thread = kDensCpt[b->dens_variant];
+        info->transform_k_step = batch_field_k_step(b->bf, b->prm.precision);
     }
     return EP_OK;
 }

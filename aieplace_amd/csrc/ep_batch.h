@@ -141,5 +141,11 @@ ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &
 ep_status batch_field_set_active(BatchField *bf, const std::vector<char> &active);
 void batch_field_free(BatchField *bf);
 void launch_field_batch(BatchField *bf, int precision, hipStream_t s);
+// Times the three K-step variants of the batched products for each precision
+// (reps launches each, after one untimed launch) and keeps the fastest; the
+// field maps must hold a valid density map.  Every variant gives the same result.
+ep_status batch_field_autotune(BatchField *bf, int reps, hipStream_t s);
+// K step (half-k staged per LDS step) the batched products use at `precision`.
+int batch_field_k_step(const BatchField *bf, int precision);
 
 }  // namespace ep

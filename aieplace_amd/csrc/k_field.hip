@@ -910,6 +910,7 @@ struct BatchField {
     uint32_t *fmax = nullptr;                // [4 * n_inst]
     int n_inst = 0;
     std::vector<int> M_of;                   // grid side of each instance
+    int tile_var[3] = {1, 1, 1};             // [precision] K-step variant (kSkBatchF32/kSkBatchSplit)
 };
 
 ep_status batch_field_init(BatchField *&out, const std::vector<BatchFieldInst> &inst)
@@ -1031,36 +1032,105 @@ void batch_field_free(BatchField *bf)
     delete bf;
 }
 
+namespace {
+
+// K-step variants of the batched products (every variant gives the same
+// result; the fastest depends on the grid sizes and the number of tiles).
+constexpr int kSkBatchF32[3] = {16, 32, 64};
+constexpr int kSkBatchSplit[3] = {32, 64, 128};
+
+int prec_index(int precision)
+{
+    return precision == EP_PREC_BF16X3 ? 1 : precision == EP_PREC_FP16X3 ? 2 : 0;
+}
+
+template <int SF, int SS>
+void launch_stage_b(int precision, int stage, dim3 grid, const GemmArgs *args, const int4 *tiles,
+                    hipStream_t s)
+{
+    const dim3 blk(256);
+    const bool inv = stage >= 2, epi = stage == 1;
+    if (precision == EP_PREC_FP32) {
+        const size_t lds = sizeof(StageF32<SF>);
+        if (inv) hipLaunchKernelGGL((k_sym_f32_b<true, 0, SF>), grid, blk, lds, s, args, tiles);
+        else if (epi) hipLaunchKernelGGL((k_sym_f32_b<false, 1, SF>), grid, blk, lds, s, args, tiles);
+        else hipLaunchKernelGGL((k_sym_f32_b<false, 0, SF>), grid, blk, lds, s, args, tiles);
+        return;
+    }
+    const size_t lds = sizeof(StageSplit<SS>);
+    if (precision == EP_PREC_BF16X3) {
+        if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, false, SS>), grid, blk, lds, s, args, tiles);
+        else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, false, SS>), grid, blk, lds, s, args, tiles);
+        else hipLaunchKernelGGL((k_sym_split_b<false, 0, false, SS>), grid, blk, lds, s, args, tiles);
+    } else {
+        if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, true, SS>), grid, blk, lds, s, args, tiles);
+        else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, true, SS>), grid, blk, lds, s, args, tiles);
+        else hipLaunchKernelGGL((k_sym_split_b<false, 0, true, SS>), grid, blk, lds, s, args, tiles);
+    }
+}
+
+}  // namespace
+
 void launch_field_batch(BatchField *bf, int precision, hipStream_t s)
 {
-    const int pi = precision == EP_PREC_BF16X3 ? 1 : precision == EP_PREC_FP16X3 ? 2 : 0;
+    const int pi = prec_index(precision);
     if (precision == EP_PREC_FP16X3)
         hipLaunchKernelGGL(k_clear_max_b, dim3((bf->n_inst + 63) / 64), dim3(64), 0, s, bf->fmax, bf->n_inst);
     for (int stage = 0; stage < 4; ++stage) {
         if (bf->n_tiles[stage] == 0) continue;
-        const dim3 grid(bf->n_tiles[stage]), blk(256);
+        const dim3 grid(bf->n_tiles[stage]);
         const GemmArgs *args = bf->d_args[pi][stage];
         const int4 *tiles = bf->d_tiles[stage];
-        const bool inv = stage >= 2, epi = stage == 1;
-        constexpr int SF = kSkF32Batch, SS = kSkSplitBatch;
-        if (precision == EP_PREC_FP32) {
-            const size_t lds = sizeof(StageF32<SF>);
-            if (inv) hipLaunchKernelGGL((k_sym_f32_b<true, 0, SF>), grid, blk, lds, s, args, tiles);
-            else if (epi) hipLaunchKernelGGL((k_sym_f32_b<false, 1, SF>), grid, blk, lds, s, args, tiles);
-            else hipLaunchKernelGGL((k_sym_f32_b<false, 0, SF>), grid, blk, lds, s, args, tiles);
-        } else {
-            const size_t lds = sizeof(StageSplit<SS>);
-            if (precision == EP_PREC_BF16X3) {
-                if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, false, SS>), grid, blk, lds, s, args, tiles);
-                else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, false, SS>), grid, blk, lds, s, args, tiles);
-                else hipLaunchKernelGGL((k_sym_split_b<false, 0, false, SS>), grid, blk, lds, s, args, tiles);
-            } else {
-                if (inv) hipLaunchKernelGGL((k_sym_split_b<true, 0, true, SS>), grid, blk, lds, s, args, tiles);
-                else if (epi) hipLaunchKernelGGL((k_sym_split_b<false, 1, true, SS>), grid, blk, lds, s, args, tiles);
-                else hipLaunchKernelGGL((k_sym_split_b<false, 0, true, SS>), grid, blk, lds, s, args, tiles);
-            }
+        switch (bf->tile_var[pi]) {
+        case 0: launch_stage_b<kSkBatchF32[0], kSkBatchSplit[0]>(precision, stage, grid, args, tiles, s); break;
+        case 1: launch_stage_b<kSkBatchF32[1], kSkBatchSplit[1]>(precision, stage, grid, args, tiles, s); break;
+        default: launch_stage_b<kSkBatchF32[2], kSkBatchSplit[2]>(precision, stage, grid, args, tiles, s); break;
         }
     }
+}
+
+ep_status batch_field_autotune(BatchField *bf, int reps, hipStream_t s)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return fail(EP_EDEVICE, "batch field autotune: hipEventCreate failed");
+    }
+    const int precs[3] = {EP_PREC_FP32, EP_PREC_BF16X3, EP_PREC_FP16X3};
+    ep_status st = EP_OK;
+    for (int pi = 0; pi < 3 && st == EP_OK; ++pi) {
+        float best = 0.f;
+        int best_v = 1;
+        for (int v = 0; v < 3; ++v) {
+            bf->tile_var[pi] = v;
+            launch_field_batch(bf, precs[pi], s);  // first launch of this variant: untimed
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps; ++r) launch_field_batch(bf, precs[pi], s);
+            (void)hipEventRecord(e1, s);
+            hipError_t e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipGetLastError();
+            if (e != hipSuccess) {
+                st = fail(EP_EDEVICE, "batch field autotune: %s", hipGetErrorString(e));
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (v == 0 || ms < best) {
+                best = ms;
+                best_v = v;
+            }
+        }
+        bf->tile_var[pi] = best_v;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+int batch_field_k_step(const BatchField *bf, int precision)
+{
+    const int v = bf->tile_var[prec_index(precision)];
+    return precision == EP_PREC_FP32 ? kSkBatchF32[v] : kSkBatchSplit[v];
 }
 
 }  // namespace ep

@@ -189,11 +189,13 @@ def batched_report(A, n=64, sample=16):
             res = bp.results()
         rate = r["instance_iterations"] / r["seconds"]
         log(f"[batched] {tag}: {r['instance_iterations']} instance-iterations in {r['seconds']:.3f}s "
-            f"= {rate:.0f}/s, {r['launches']} replays, density tile {r['density_cells_per_thread']} cells/thread")
+            f"= {rate:.0f}/s, {r['launches']} replays, density tile {r['density_cells_per_thread']} cells/thread, "
+            f"transform K step {r['transform_k_step']}")
         rep[tag] = dict(instance_iterations_per_s=round(rate, 1), seconds=round(r["seconds"], 4),
                         instance_iterations=int(r["instance_iterations"]), evals=int(r["evals"]),
                         finished=int(r["finished"]), replays=int(r["launches"]),
                         density_cells_per_thread=int(r["density_cells_per_thread"]),
+                        transform_k_step=int(r["transform_k_step"]),
                         max_overflow=round(float(res["overflow"].max()), 4))
         if prec == api.EP_PREC_FP32:
             hpwl32 = res["hpwl"].copy()

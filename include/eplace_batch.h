@@ -7,7 +7,8 @@
  * (instance, tile) work items, and a device-side control kernel applies each
  * instance's backtracking decision and lambda/gamma schedules without a host
  * round trip.  The host polls completion every few replays.  Tile shapes (density
- * cells per block) are picked by a timing autotuner at creation.
+ * cells per block, the transform products' K step) are picked by a timing
+ * autotuner at creation.
  * No counterpart in the reference (single instance, AIEplace.cpp:44-76).
  */
 #ifndef EPLACE_BATCH_H
@@ -31,6 +32,8 @@ typedef struct ep_batch_run_info {
     int32_t launches;           /* graph replays                                    */
     double seconds;             /* device-synchronised wall time of the loop         */
     int32_t density_cells_per_thread;  /* autotuned scatter tile                    */
+    int32_t transform_k_step;   /* autotuned K step of the transform products (half-k
+                                   per LDS stage) at the batch's precision           */
 } ep_batch_run_info;
 
 /* n instances (1..1024); bins[i] = square bin grid of instance i (multiple of 32).

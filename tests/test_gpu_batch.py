@@ -89,6 +89,11 @@ def test_batch_trajectory_matches_single(precision):
         info = bp.run()
         res = bp.results()
     assert info["finished"] == len(counts)
+    # the autotuned K step of the batched products (the single-instance
+    # products use another one: equal trajectories show that it changes nothing)
+    steps = {16, 32, 64} if precision == api.EP_PREC_FP32 else {32, 64, 128}
+    say(f"  transform K step {info['transform_k_step']}, density tile {info['density_cells_per_thread']}")
+    assert info["transform_k_step"] in steps
     for i, (nl, b) in enumerate(zip(nls, bins)):
         with A.Placer(nl, single_params(p, b)) as pl:
             r = pl.run()
