@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libeplace.so")
 
-EP_OK, EP_EINVAL, EP_EDEVICE, EP_ENAN, EP_ENOMEM, EP_ESTATE, EP_ENOTSUP = range(7)
+EP_OK, EP_EINVAL, EP_EDEVICE, EP_ENAN, EP_ENOMEM, EP_ESTATE, EP_ENOTSUP, EP_EIO = range(8)
 EP_PREC_FP32, EP_PREC_BF16X3, EP_PREC_FP16X3 = 0, 1, 2
 EP_GEN_MESH, EP_GEN_STDCELL = 0, 1
 
@@ -79,7 +79,7 @@ class EpDesign(C.Structure):
     _fields_ = [("nl", EpNetlist), ("mesh_side", C.c_int32)]
 
 
-# every symbol include/eplace.h and include/eplace_gen.h declare
+# every symbol the headers under include/ declare
 EXPORTS = [
     "ep_default_params", "ep_create", "ep_plan_fillers", "ep_destroy", "ep_last_error", "ep_get_sizes",
     "ep_get_cells", "ep_get_nets", "ep_set_positions", "ep_get_positions", "ep_wa_grad",
@@ -91,6 +91,8 @@ EXPORTS = [
     "ep_batch_create", "ep_batch_destroy", "ep_batch_reset", "ep_batch_init", "ep_batch_run",
     "ep_batch_results", "ep_batch_sizes", "ep_batch_positions", "ep_batch_gradient",
     "ep_batch_state",
+    "ep_lefdef_read", "ep_lefdef_read_dir", "ep_lefdef_free", "ep_lefdef_netlist", "ep_lefdef_get_info",
+    "ep_lefdef_write_def",
 ]
 
 MLP_IN, MLP_HID = 16, 64
@@ -109,6 +111,13 @@ class EpBatchRunInfo(C.Structure):
                 ("launches", C.c_int32), ("seconds", C.c_double),
                 ("density_cells_per_thread", C.c_int32),
                 ("transform_k_step", C.c_int32)]
+
+
+class EpLefdefInfo(C.Structure):
+    _fields_ = [("units", C.c_int32), ("n_macros", C.c_int32), ("n_components", C.c_int32),
+                ("n_io_pins", C.c_int32), ("n_nets", C.c_int32), ("n_mov", C.c_int32),
+                ("n_fix", C.c_int32), ("view_nets", C.c_int32), ("view_pins", C.c_int32),
+                ("die", C.c_double * 4)]
 
 
 class EplaceError(RuntimeError):
@@ -176,6 +185,12 @@ def _load():
     L.ep_batch_positions.argtypes = [vp, C.c_int32, _fp, _fp]
     L.ep_batch_gradient.argtypes = [vp, C.c_int32, _fp, _fp]
     L.ep_batch_state.argtypes = [vp, C.c_int32, _dp, _dp, _dp]
+    L.ep_lefdef_read.argtypes = [C.POINTER(C.c_char_p), C.c_int32, C.c_char_p, C.POINTER(vp)]
+    L.ep_lefdef_read_dir.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.ep_lefdef_free.argtypes = [vp]
+    L.ep_lefdef_netlist.argtypes = [vp, C.POINTER(EpNetlist)]
+    L.ep_lefdef_get_info.argtypes = [vp, C.POINTER(EpLefdefInfo), C.c_char_p]
+    L.ep_lefdef_write_def.argtypes = [vp, _fp, _fp, C.c_char_p]
     return L
 
 

@@ -33,7 +33,8 @@ enum {
     EP_ENAN = 3,     /* non-finite value detected in positions/gradients  */
     EP_ENOMEM = 4,   /* device or host allocation failed                  */
     EP_ESTATE = 5,   /* call out of order (e.g. ep_step before init)      */
-    EP_ENOTSUP = 6   /* feature not available in this build/config        */
+    EP_ENOTSUP = 6,  /* feature not available in this build/config        */
+    EP_EIO = 7       /* file missing, unreadable or malformed (LEF/DEF)   */
 };
 
 /* Arithmetic of the field-solve transform products. */

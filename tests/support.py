@@ -35,3 +35,38 @@ def oracle_params(params: api.EpParams, bw: float, n_nets: int) -> dict:
                 hpwl_ref=float(href), mu_max=params.mu_max, mu_min=params.mu_min,
                 max_backtrack=params.max_backtrack, stop_overflow=params.stop_overflow,
                 max_iters=params.max_iters, init_disp_bins=params.init_disp_bins)
+
+
+def netlist_lefdef(nl, units=1000, name="gen"):
+    """LEF (one macro per cell) and DEF text for a netlist; cells [movable | fixed]."""
+    lef, comps, nets = [], [], []
+    q = lambda v: int(round(float(v) * units))
+    n_all = len(nl.w)
+    pin_names = {}
+    for n in range(nl.n_nets):
+        for p in range(nl.net_start[n], nl.net_start[n + 1]):
+            pin_names.setdefault(int(nl.pin_cell[p]), []).append(p)
+    for k in range(n_all):
+        lef.append(f"MACRO M{k}\n  SIZE {float(nl.w[k])!r} BY {float(nl.h[k])!r} ;")
+        for p in pin_names.get(k, []):
+            dx, dy = float(nl.pin_dx[p]), float(nl.pin_dy[p])
+            lef.append(f"  PIN P{p} PORT LAYER m1 ; RECT {dx - 0.01!r} {dy - 0.01!r} {dx + 0.01!r} {dy + 0.01!r} ; "
+                       f"END END P{p}")
+        lef.append(f"END M{k}")
+        st = "PLACED" if k < nl.n_mov else "FIXED"
+        comps.append(f"- c{k} M{k} + {st} ( {q(nl.x[k])} {q(nl.y[k])} ) N ;")
+    for n in range(nl.n_nets):
+        conn = " ".join(f"( c{nl.pin_cell[p]} P{p} )" for p in range(nl.net_start[n], nl.net_start[n + 1]))
+        nets.append(f"- net{n} {conn} ;")
+    die = " ".join(str(q(v)) for v in nl.die)
+    dx0, dy0, dx1, dy1 = die.split()
+    d = (f"DESIGN {name} ;\nUNITS DISTANCE MICRONS {units} ;\nDIEAREA ( {dx0} {dy0} ) ( {dx1} {dy1} ) ;\n"
+         f"COMPONENTS {n_all} ;\n" + "\n".join(comps) + "\nEND COMPONENTS\n"
+         f"NETS {nl.n_nets} ;\n" + "\n".join(nets) + "\nEND NETS\nEND DESIGN\n")
+    return "\n".join(lef) + "\nEND LIBRARY\n", d
+
+
+def round_half_away(v):
+    """Rounding to the nearest integer, ties away from zero (C llround): how
+    positions are converted to DEF database units."""
+    return np.sign(v) * np.floor(np.abs(v) + 0.5)
