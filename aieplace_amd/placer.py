@@ -201,6 +201,30 @@ class Placer:
                       for l in logs[:min(log_cap, ri.iterations)]]
         return out
 
+    def run_with_report(self, report, design=None, max_iters: Optional[int] = None) -> dict:
+        """Placer::run with its operator output (AIEplace.cpp:44-76, 1048-1124): one
+        host-controlled Nesterov step per iteration (ep_step), each reported to
+        `report` (an aieplace_amd.report.RunReport: table row, snapshot every
+        report.snapshot_every iterations), then statistics.md, the final snapshot
+        and, for a LEF/DEF design, its DEF."""
+        import time
+        limit = self.params.max_iters if max_iters is None else max_iters
+        t0 = time.perf_counter()
+        self.initializePlacement()
+        rec, trials = None, 0
+        while self.iteration < limit:
+            rec = self.nudgeAllNodes()
+            trials += rec["evals"]
+            report.iteration(self, rec)
+            if rec["overflow"] <= self.params.stop_overflow:
+                break
+        # evals: the Nesterov trials' gradient evaluations (the set-up's are not counted)
+        res = dict(iterations=self.iteration, evals=trials,
+                   hpwl=rec["hpwl"] if rec else self.hpwl(), overflow=rec["overflow"] if rec else self.overflow(),
+                   seconds=time.perf_counter() - t0)
+        res["statistics"] = report.final(self, res, self.params, design)
+        return res
+
     def printFinalResults(self, res: dict):
         print(f"Iterations {res['iterations']}  Final HPWL {res['hpwl']:.6e}  "
               f"Final Overflow {res['overflow']:.4f}  ({res['seconds']:.3f} s)")

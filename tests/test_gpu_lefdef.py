@@ -7,6 +7,7 @@ ep_lefdef_write_def writes the placed positions back: re-read, they equal the
 placer's positions rounded to database units, fixed cells and nets unchanged,
 and the HPWL of the re-read design equals the placer's to that rounding.
 """
+import os
 import sys
 
 import numpy as np
@@ -75,3 +76,37 @@ def test_lefdef_design_places_and_writes_back(tmp_path, oracle):
     assert h_placed == pytest.approx(h_dev, rel=1e-6)
     # each pin moves by at most half a database unit per axis
     assert abs(h_back - h_placed) <= 2 * 0.0005 * nl.n_pins
+
+
+def test_run_with_report_writes_the_run_outputs(tmp_path):
+    """Placer.run_with_report on a LEF/DEF design: the operator output of the
+    reference's run (statistics.md, placement_<it>.png every 10 iterations and
+    at the end, <design>.def), and the same run as the device-controlled ep_run
+    (host- and device-side control take the same decisions)."""
+    from aieplace_amd.report import RunReport
+    say("case report mesh 576 cells")
+    gen = A.generate("mesh", seed=1)
+    lef, df = netlist_lefdef(gen, name="mesh576")
+    (tmp_path / "cells.lef").write_text(lef)
+    (tmp_path / "floorplan.def").write_text(df)
+    p = A.placer_params(A.CONFIGS["mesh_c1"])
+    with LefDef.from_dir(str(tmp_path)) as d:
+        nl = d.netlist()
+        with A.Placer(nl, p) as pl:
+            rep = RunReport("mesh576", out_dir=str(tmp_path / "run"), snapshot_every=10, pixels=256, quiet=True)
+            res = pl.run_with_report(rep, design=d)
+        with A.Placer(nl, p) as pl2:
+            dev = pl2.run(log_cap=10_000)
+    say(f"  reported: it={res['iterations']} hpwl={res['hpwl']:.6e} tau={res['overflow']:.4f}; "
+        f"ep_run: it={dev['iterations']} hpwl={dev['hpwl']:.6e}")
+    assert res["overflow"] <= p.stop_overflow
+    assert res["iterations"] == dev["iterations"]
+    assert res["evals"] == sum(l["evals"] for l in dev["log"])
+    assert res["hpwl"] == pytest.approx(dev["hpwl"], rel=1e-9)
+    files = set(os.listdir(tmp_path / "run"))
+    want = {f"placement_{k}.png" for k in range(0, res["iterations"], 10)} | {
+        f"placement_{res['iterations']}.png", "statistics.md", "mesh576.def"}
+    assert want <= files, sorted(want - files)
+    stats = (tmp_path / "run" / "statistics.md").read_text()
+    assert f"| Iterations | {res['iterations']} |" in stats
+    assert f"| Final HPWL | {res['hpwl']:.6e} |" in stats
