@@ -92,7 +92,7 @@ EXPORTS = [
     "ep_batch_results", "ep_batch_sizes", "ep_batch_positions", "ep_batch_gradient",
     "ep_batch_state",
     "ep_lefdef_read", "ep_lefdef_read_dir", "ep_lefdef_free", "ep_lefdef_netlist", "ep_lefdef_get_info",
-    "ep_lefdef_write_def",
+    "ep_lefdef_write_def", "ep_legalize",
 ]
 
 MLP_IN, MLP_HID = 16, 64
@@ -118,6 +118,11 @@ class EpLefdefInfo(C.Structure):
                 ("n_io_pins", C.c_int32), ("n_nets", C.c_int32), ("n_mov", C.c_int32),
                 ("n_fix", C.c_int32), ("view_nets", C.c_int32), ("view_pins", C.c_int32),
                 ("die", C.c_double * 4)]
+
+
+class EpLegalInfo(C.Structure):
+    _fields_ = [("total_disp", C.c_double), ("max_disp", C.c_double), ("sum_sq_disp", C.c_double),
+                ("rows", C.c_int32), ("segments", C.c_int32), ("seconds", C.c_double)]
 
 
 class EplaceError(RuntimeError):
@@ -191,6 +196,8 @@ def _load():
     L.ep_lefdef_netlist.argtypes = [vp, C.POINTER(EpNetlist)]
     L.ep_lefdef_get_info.argtypes = [vp, C.POINTER(EpLefdefInfo), C.c_char_p]
     L.ep_lefdef_write_def.argtypes = [vp, _fp, _fp, C.c_char_p]
+    L.ep_legalize.argtypes = [C.POINTER(EpNetlist), _fp, _fp, C.c_float, C.c_float, C.c_int32, _fp, _fp,
+                              C.POINTER(EpLegalInfo)]
     return L
 
 

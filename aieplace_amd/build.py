@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libeplace.so")
 RUNNER = os.path.join(HERE, "eplace_run")
 
 SOURCES = ["k_place.hip", "k_field.hip", "k_mlp.hip", "k_batch.hip", "ep_api.cpp", "batch.cpp",
-           "generator.cpp", "lefdef.cpp"]
+           "generator.cpp", "lefdef.cpp", "legalize.cpp"]
 ARCH = os.environ.get("EPLACE_OFFLOAD_ARCH", "gfx950")
 
 

@@ -70,3 +70,26 @@ def round_half_away(v):
     """Rounding to the nearest integer, ties away from zero (C llround): how
     positions are converted to DEF database units."""
     return np.sign(v) * np.floor(np.abs(v) + 0.5)
+
+
+def check_legal(nl, x, y, row, site, tol=1e-4):
+    n = nl.n_mov
+    xl, yl, xh, yh = nl.die
+    w, h = nl.w[:n].astype(np.float64), nl.h[:n].astype(np.float64)
+    x, y = np.asarray(x, np.float64), np.asarray(y, np.float64)
+    r = (y - yl) / row
+    s = (x - xl) / site
+    assert np.abs(r - np.rint(r)).max() < tol, "off the rows"
+    assert np.abs(s - np.rint(s)).max() < tol, "off the sites"
+    assert x.min() >= xl - tol and (x + w).max() <= xh + tol and y.min() >= yl - tol and (y + h).max() <= yh + tol
+    rows = np.rint(r).astype(int)
+    for rr in np.unique(rows):
+        idx = np.where(rows == rr)[0]
+        idx = idx[np.argsort(x[idx])]
+        assert (x[idx[1:]] >= x[idx[:-1]] + w[idx[:-1]] - tol).all(), f"overlap in row {rr}"
+    fx, fy = nl.x[n:].astype(np.float64), nl.y[n:].astype(np.float64)
+    fw, fh = nl.w[n:].astype(np.float64), nl.h[n:].astype(np.float64)
+    for k in range(len(fx)):
+        ox = np.minimum(x + w, fx[k] + fw[k]) - np.maximum(x, fx[k])
+        oy = np.minimum(y + h, fy[k] + fh[k]) - np.maximum(y, fy[k])
+        assert not ((ox > tol) & (oy > tol)).any(), f"overlap with fixed cell {k}"

@@ -110,3 +110,27 @@ def test_run_with_report_writes_the_run_outputs(tmp_path):
     stats = (tmp_path / "run" / "statistics.md").read_text()
     assert f"| Iterations | {res['iterations']} |" in stats
     assert f"| Final HPWL | {res['hpwl']:.6e} |" in stats
+
+
+def test_global_placement_then_legalisation():
+    """The flow after global placement: the placed 20k-cell design legalised on
+    the host (Abacus) is legal, and its HPWL grows by a few percent."""
+    from aieplace_amd.legalize import legalize
+    from support import check_legal
+    say("case legalise stdcell n=20000 bins=128")
+    nl = A.generate("stdcell", seed=5, n_movable=20_000)
+    p = A.placer_params(A.CONFIGS["stdcell_200k"], bins_x=128, bins_y=128)
+    with A.Placer(nl, p) as pl:
+        r = pl.run()
+        x, y = pl.get_positions()
+        h_global = pl.hpwl()
+    lx, ly, info = legalize(nl, x, y)
+    check_legal(nl, lx, ly, 1.0, 1.0)
+    placed = A.Netlist(w=nl.w, h=nl.h, x=np.concatenate([lx, nl.x[nl.n_mov:]]),
+                       y=np.concatenate([ly, nl.y[nl.n_mov:]]), n_mov=nl.n_mov, net_start=nl.net_start,
+                       pin_cell=nl.pin_cell, pin_dx=nl.pin_dx, pin_dy=nl.pin_dy, die=nl.die)
+    h_legal = hpwl(placed)
+    say(f"  global: tau={r['overflow']:.4f} hpwl={h_global:.6e}; legal: hpwl={h_legal:.6e} "
+        f"(+{100 * (h_legal / h_global - 1):.2f}%), mean disp {info['total_disp'] / nl.n_mov:.3f}, "
+        f"max {info['max_disp']:.2f}, {info['seconds'] * 1e3:.1f} ms")
+    assert h_legal < 1.2 * h_global
