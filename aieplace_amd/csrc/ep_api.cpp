@@ -1144,7 +1144,8 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
     hipGraph_t g = nullptr;
     EP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int t = 0; t < kCtlTrialsPerGraph; ++t) {
-        launch_mlp_propose(x, s);
+        // the model's proposal for this trial was made by the previous decision
+        // (k_finalize) or by k_step_begin
         launch_trial(x, vc, uc, gc, vn, un, true, s);
         launch_sort(x, vt, true, s);
         if (x.n_place >= kForkMinCells) {

@@ -12,6 +12,7 @@
 
 #include "../../include/eplace_mlp.h"
 #include "ep_ctx.h"
+#include "ep_mlp.h"
 
 namespace ep {
 
@@ -132,30 +133,13 @@ __global__ void k_mlp_adam(float *__restrict__ w, const float *__restrict__ g, f
     w[p] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-// In-loop proposal: 64 threads evaluate the model on the feature vector staged
-// in the device scalars and overwrite the trial step when use_model is set.
+// In-loop proposal (host-controlled steps): the model on the feature vector
+// staged in the device scalars overwrites the trial step when use_model is set.
+// Device-controlled runs make the same proposal inside k_finalize's decision.
 __global__ __launch_bounds__(MH) void k_mlp_propose(const float *__restrict__ w,
                                                     DevScalars *__restrict__ sc)
 {
-    __shared__ float xn[MI], a1[MH], a2[MH];
-    if (!sc->use_model || sc->halt) return;  // uniform across the block
-    const int j = threadIdx.x;
-    if (j < MI) xn[j] = (sc->feat[j] - w[j]) / w[16 + j];
-    __syncthreads();
-    float acc = w[OFF_B1 + j];
-    for (int i = 0; i < MI; ++i) acc += w[OFF_W1 + j * MI + i] * xn[i];
-    a1[j] = gelu(acc);
-    __syncthreads();
-    acc = w[OFF_B2 + j];
-    for (int k = 0; k < MH; ++k) acc += w[OFF_W2 + j * MH + k] * a1[k];
-    a2[j] = gelu(acc);
-    __syncthreads();
-    if (j == 0) {
-        float o = w[OFF_B3];
-        for (int k = 0; k < MH; ++k) o += w[OFF_W3 + k] * a2[k];
-        const float alpha = exp10f(o * w[33] + w[32]) * sc->step_unit;
-        if (isfinite(alpha) && alpha > 0.0f) sc->alpha = alpha;  // else keep the Lipschitz step
-    }
+    mlp_propose_block(w, sc->feat, sc, sc->use_model && !sc->halt);
 }
 
 struct Dev {

@@ -14,6 +14,7 @@
 #include <climits>
 
 #include "ep_ctx.h"
+#include "ep_mlp.h"
 #include "ep_device.h"
 #include "ep_sched.h"
 
@@ -387,6 +388,7 @@ struct StepCtlArgs {
     ep_step_info *log;
     float *rec;
     CtlParams p;
+    const float *mlp_w;     // learned step: the model proposes the next trial's step
 };
 
 __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
@@ -435,29 +437,40 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     if (threadIdx.x < B)
         for (int k = 0; k < 10; ++k) sfin[threadIdx.x][k] = load_agent(&fin[threadIdx.x * 10 + k]);
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    double t[10];
-    for (int k = 0; k < 10; ++k) t[k] = sfin[0][k];
-    for (int j = 1; j < B; ++j) {
-        for (int k = 0; k < 9; ++k) t[k] += sfin[j][k];
-        t[9] = fmax(t[9], sfin[j][9]);
+    __shared__ int propose;
+    __shared__ float feat[16];  // the decision thread's features, for the whole block
+    if (threadIdx.x == 0) {
+        double t[10];
+        for (int k = 0; k < 10; ++k) t[k] = sfin[0][k];
+        for (int j = 1; j < B; ++j) {
+            for (int k = 0; k < 9; ++k) t[k] += sfin[j][k];
+            t[9] = fmax(t[9], sfin[j][9]);
+        }
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // next launch
+        DevResult R;
+        R.hpwl = t[0];
+        R.wa = t[1];
+        R.ovf_num = t[2];
+        R.tau = mov_area > 0 ? t[2] / mov_area : 0.0;
+        R.dv2 = t[3];
+        R.dg2 = t[4];
+        R.sum_gw = t[5];
+        R.sum_f = t[6];
+        R.gnorm2 = t[7];
+        R.nonfinite = (int64_t)t[8];
+        R.gmax = t[9];
+        R.alpha_used = sc->alpha;
+        *res = R;
+        if (ctl.ctl) step_decide(ctl.ctl, R, sc, ctl.p, ctl.log, ctl.rec);  // device-controlled run
+        propose = ctl.mlp_w && sc->use_model && !sc->halt;
+        if (propose)
+            for (int k = 0; k < 16; ++k) feat[k] = sc->feat[k];
     }
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // next launch
-    DevResult R;
-    R.hpwl = t[0];
-    R.wa = t[1];
-    R.ovf_num = t[2];
-    R.tau = mov_area > 0 ? t[2] / mov_area : 0.0;
-    R.dv2 = t[3];
-    R.dg2 = t[4];
-    R.sum_gw = t[5];
-    R.sum_f = t[6];
-    R.gnorm2 = t[7];
-    R.nonfinite = (int64_t)t[8];
-    R.gmax = t[9];
-    R.alpha_used = sc->alpha;
-    *res = R;
-    if (ctl.ctl) step_decide(ctl.ctl, R, sc, ctl.p, ctl.log, ctl.rec);  // device-controlled run
+    // the model's step for the next trial, while this block is still resident
+    if (ctl.mlp_w) {
+        __syncthreads();
+        mlp_propose_block(ctl.mlp_w, feat, sc, propose != 0);
+    }
 }
 
 // ------------------------------------------------------------------ Nesterov
@@ -697,14 +710,25 @@ __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
     *ctl = S;
 }
 
-// Start of the first iteration of a device-controlled run.
-__global__ void k_step_begin(DevCtl *__restrict__ ctl, DevScalars *__restrict__ sc, CtlParams p)
+// Start of the first iteration of a device-controlled run (and the model's
+// proposal for its first trial).
+__global__ void k_step_begin(DevCtl *__restrict__ ctl, DevScalars *__restrict__ sc, CtlParams p,
+                             const float *__restrict__ mlp_w)
 {
-    if (threadIdx.x != 0) return;
-    DevCtl S = *ctl;
-    if (S.done) return;
-    ctl_begin(S, sc, p);
-    *ctl = S;
+    __shared__ int propose;
+    __shared__ float feat[16];
+    if (threadIdx.x == 0) {
+        DevCtl S = *ctl;
+        propose = 0;
+        if (!S.done) {
+            ctl_begin(S, sc, p);
+            *ctl = S;
+            propose = mlp_w && sc->use_model && !sc->halt;
+            for (int k = 0; k < 16; ++k) feat[k] = sc->feat[k];
+        }
+    }
+    __syncthreads();
+    if (mlp_w) mlp_propose_block(mlp_w, feat, sc, propose != 0);
 }
 
 GeomArgs geom_args(const Ctx &c)
@@ -808,7 +832,7 @@ void launch_finalize(Ctx &c, hipStream_t s)
 
 void launch_finalize_decide(Ctx &c, const CtlParams &p, hipStream_t s)
 {
-    const StepCtlArgs a{c.d_ctl, c.d_log, c.d_rec, p};
+    const StepCtlArgs a{c.d_ctl, c.d_log, c.d_rec, p, p.learned ? (const float *)c.d_mlp_w : nullptr};
     hipLaunchKernelGGL(k_finalize, dim3(kFinalizeBlocks), dim3(kFinalizeThreads), 0, s, c.part,
                        c.part_wa_off, c.wa_part_blocks, c.part_ovf_off, c.ovf_blocks, c.part_cell_off,
                        c.cell_blocks, c.mov_area, c.d_scal, c.d_res, c.fin, c.fin_ticket, a);
@@ -825,7 +849,8 @@ void launch_trial(Ctx &c, Pair<const float2> v, Pair<const float2> u, Pair<const
 
 void launch_step_begin(Ctx &c, const CtlParams &p, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, c.d_ctl, c.d_scal, p);
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(kMlpHid), 0, s, c.d_ctl, c.d_scal, p,
+                       p.learned ? (const float *)c.d_mlp_w : nullptr);
 }
 
 void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipStream_t s)
