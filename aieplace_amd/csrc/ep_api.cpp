@@ -88,7 +88,8 @@ void push_scalars(Ctx &c, hipStream_t s)
 }
 
 // One gradient evaluation at `pos`: density + overflow, WA, field, gather.
-// `counted`: a counting launch_trial produced pos (its sort counts are made).
+// `counted`: a counting launch_trial produced pos from vref (its sort counts
+// and its |pos - vref|^2 partial sums are made).
 void enqueue_eval(Ctx &c, const float2 *pos, const float2 *vref, const float2 *gref, float2 *gout,
                   float2 *gw_out, float2 *f_out, hipStream_t s, bool counted = false)
 {
@@ -97,8 +98,8 @@ void enqueue_eval(Ctx &c, const float2 *pos, const float2 *vref, const float2 *g
     launch_overflow(c, s);
     launch_wa(c, pos, s);
     launch_field(c, c.prm.precision, s);
-    launch_cell_grad(c, pos, vref, gref, gout, gw_out, f_out, s);
-    launch_finalize(c, s);
+    launch_cell_grad(c, pos, counted ? nullptr : vref, gref, gout, gw_out, f_out, s);
+    launch_finalize(c, s, counted);
     (void)hipMemcpyAsync(c.h_res, c.d_res, sizeof(DevResult), hipMemcpyDeviceToHost, s);
 }
 
@@ -535,7 +536,9 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     c->part_wa_off = 0;
     c->part_ovf_off = 2 * c->wa_part_blocks;
     c->part_cell_off = c->part_ovf_off + 4 * c->ovf_blocks;
-    c->part_total = c->part_cell_off + 7 * c->cell_blocks;
+    c->trial_blocks = (int)(((int64_t)c->n_place + kBlock * kSortCellsPerThread - 1) / (kBlock * kSortCellsPerThread));
+    c->part_trial_off = c->part_cell_off + 7 * c->cell_blocks;
+    c->part_total = c->part_trial_off + c->trial_blocks;
 #define EP_TRY(x)                 \
     do {                          \
         ep_status s_ = (x);       \
@@ -1141,6 +1144,7 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
     const Pair<const float2> vc{{x.V[0], x.V[1]}}, uc{{x.U[0], x.U[1]}}, gc{{x.G[0], x.G[1]}};
     const Pair<float2> vn{{x.V[1], x.V[0]}}, un{{x.U[1], x.U[0]}}, gn{{x.G[1], x.G[0]}};
     const Pair<const float2> vt{{x.V[1], x.V[0]}};
+    const Pair<const float2> none{{nullptr, nullptr}};  // |v' - v|^2 comes from k_trial
     hipGraph_t g = nullptr;
     EP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int t = 0; t < kCtlTrialsPerGraph; ++t) {
@@ -1160,13 +1164,13 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
             launch_overflow(x, s);
             launch_field(x, x.prm.precision, s);
             EP_HIP(hipStreamWaitEvent(s, x.join, 0));
-            launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s, x.gw_cell);
+            launch_cell_grad(x, vt, none, gc, gn, nullptr, nullptr, s, x.gw_cell);
         } else {
             launch_density(x, s);
             launch_overflow(x, s);
             launch_wa(x, vt, s);
             launch_field(x, x.prm.precision, s);
-            launch_cell_grad(x, vt, vc, gc, gn, nullptr, nullptr, s);
+            launch_cell_grad(x, vt, none, gc, gn, nullptr, nullptr, s);
         }
         launch_finalize_decide(x, p, s);
     }
@@ -1347,9 +1351,9 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
             else if (nm == "overflow") launch_overflow(*c, c->stream);
             else if (nm == "wa_grad") launch_wa(*c, c->V[a], c->stream);
             else if (nm == "field_transforms") launch_field(*c, c->prm.precision, c->stream);
-            else if (nm == "cell_grad") launch_cell_grad(*c, c->V[a], c->V[a], c->G[a], c->G[b], nullptr, nullptr, c->stream);
+            else if (nm == "cell_grad") launch_cell_grad(*c, c->V[a], nullptr, c->G[a], c->G[b], nullptr, nullptr, c->stream);
             else if (nm == "nesterov_trial") launch_trial(*c, c->V[a], c->U[a], c->G[a], c->V[b], c->U[b], true, c->stream);
-            else launch_finalize(*c, c->stream);
+            else launch_finalize(*c, c->stream, true);
         }
         EP_HIP(hipEventRecord(e1, c->stream));
         // the counting trials left tile counts behind: consume them (the scan

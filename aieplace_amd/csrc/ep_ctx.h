@@ -195,7 +195,9 @@ void launch_cell_grad(Ctx &c, Pair<const float2> pos, Pair<const float2> vref,
                       hipStream_t s, const float2 *gw_cell = nullptr);
 // gw_cell[c] = sum of movable cell c's pin partials (after launch_wa).
 void launch_pin_sum(Ctx &c, hipStream_t s);
-void launch_finalize(Ctx &c, hipStream_t s);
+// trial: the evaluated point came from a counting launch_trial, whose partial
+// sums hold |v' - v|^2 (k_cell_grad then gets no vref)
+void launch_finalize(Ctx &c, hipStream_t s, bool trial = false);
 // Nesterov trial into v_new/u_new; `count`: also count v_new into the sort's
 // tiles (the following evaluation then sorts with counted = true).
 void launch_trial(Ctx &c, Pair<const float2> v, Pair<const float2> u, Pair<const float2> g,
@@ -279,8 +281,8 @@ struct Ctx {
     double *part = nullptr;   // partial sums, see part_* offsets
     double *fin = nullptr;    // second-level sums of the finalize blocks (32 x 10)
     unsigned *fin_ticket = nullptr;  // finalize blocks done (0 between launches)
-    int32_t part_wa_off = 0, part_ovf_off = 0, part_cell_off = 0, part_total = 0;
-    int32_t wa_part_blocks = 0, ovf_blocks = 0, cell_blocks = 0;
+    int32_t part_wa_off = 0, part_ovf_off = 0, part_cell_off = 0, part_trial_off = 0, part_total = 0;
+    int32_t wa_part_blocks = 0, ovf_blocks = 0, cell_blocks = 0, trial_blocks = 0;
     DevScalars *d_scal = nullptr, *h_scal = nullptr;  // device / pinned host
     DevResult *d_res = nullptr, *h_res = nullptr;
 

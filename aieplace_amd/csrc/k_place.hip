@@ -397,7 +397,7 @@ __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
 
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     const double *__restrict__ part, int wa_off, int wa_blocks, int ovf_off, int ovf_blocks,
-    int cell_off, int cell_blocks, double mov_area, DevScalars *__restrict__ sc,
+    int cell_off, int cell_blocks, int trial_off, int trial_blocks, double mov_area, DevScalars *__restrict__ sc,
     DevResult *__restrict__ res, double *__restrict__ fin, unsigned *__restrict__ ticket,
     StepCtlArgs ctl)
 {
@@ -422,6 +422,8 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
         v[3] += q[0]; v[4] += q[1]; v[5] += q[2]; v[6] += q[3]; v[7] += q[4]; v[8] += q[5];
         v[9] = fmax(v[9], q[6]);
     }
+    slice(trial_blocks, lo, hi);  // |v' - v|^2 of a trial (k_cell_grad's sum is then 0)
+    for (int i = lo + threadIdx.x; i < hi; i += kFinalizeThreads) v[3] += part[trial_off + i];
     __shared__ double out[10];
     __shared__ int last;
     block_reduce<10, 1, kFinalizeThreads>(v, out);
@@ -487,7 +489,8 @@ __global__ __launch_bounds__(kBlock) void k_trial(Pair<const float2> v_pair,
                                                   const DevScalars *__restrict__ sc,
                                                   Pair<float2> v_new_pair,
                                                   Pair<float2> u_new_pair,
-                                                  int32_t *__restrict__ count)
+                                                  int32_t *__restrict__ count,
+                                                  double *__restrict__ dv2_part)
 {
     if (sc->halt) return;
     const int sel = sc->sel;
@@ -503,6 +506,7 @@ __global__ __launch_bounds__(kBlock) void k_trial(Pair<const float2> v_pair,
     }
     const float alpha = sc->alpha, coef = sc->coef;
     const int base = blockIdx.x * kBlock * kSortCellsPerThread;
+    double dv2[1] = {0.0};  // |v' - v|^2, for the step-length estimate
 #pragma unroll
     for (int k = 0; k < kSortCellsPerThread; ++k) {
         const int c = base + k * kBlock + threadIdx.x;
@@ -514,8 +518,11 @@ __global__ __launch_bounds__(kBlock) void k_trial(Pair<const float2> v_pair,
             u_new[c] = un;
             v_new[c] = vn;
             if (count) atomicAdd(&hist[cell_key(vn, s, g, Mx, My)], 1);
+            const double dx = (double)vn.x - vv.x, dy = (double)vn.y - vv.y;
+            dv2[0] += dx * dx + dy * dy;
         }
     }
+    if (dv2_part) block_reduce<1, 0>(dv2, dv2_part + blockIdx.x);
     if (count) {
         __syncthreads();
         for (int t = threadIdx.x; t < kSortTiles; t += kBlock)
@@ -822,12 +829,13 @@ void launch_cell_grad(Ctx &c, Pair<const float2> pos, Pair<const float2> vref,
                        c.part + c.part_cell_off, gw_cell);
 }
 
-void launch_finalize(Ctx &c, hipStream_t s)
+void launch_finalize(Ctx &c, hipStream_t s, bool trial)
 {
     StepCtlArgs none{};
     hipLaunchKernelGGL(k_finalize, dim3(kFinalizeBlocks), dim3(kFinalizeThreads), 0, s, c.part,
                        c.part_wa_off, c.wa_part_blocks, c.part_ovf_off, c.ovf_blocks, c.part_cell_off,
-                       c.cell_blocks, c.mov_area, c.d_scal, c.d_res, c.fin, c.fin_ticket, none);
+                       c.cell_blocks, c.part_trial_off, trial ? c.trial_blocks : 0, c.mov_area, c.d_scal,
+                       c.d_res, c.fin, c.fin_ticket, none);
 }
 
 void launch_finalize_decide(Ctx &c, const CtlParams &p, hipStream_t s)
@@ -835,7 +843,8 @@ void launch_finalize_decide(Ctx &c, const CtlParams &p, hipStream_t s)
     const StepCtlArgs a{c.d_ctl, c.d_log, c.d_rec, p, p.learned ? (const float *)c.d_mlp_w : nullptr};
     hipLaunchKernelGGL(k_finalize, dim3(kFinalizeBlocks), dim3(kFinalizeThreads), 0, s, c.part,
                        c.part_wa_off, c.wa_part_blocks, c.part_ovf_off, c.ovf_blocks, c.part_cell_off,
-                       c.cell_blocks, c.mov_area, c.d_scal, c.d_res, c.fin, c.fin_ticket, a);
+                       c.cell_blocks, c.part_trial_off, c.trial_blocks, c.mov_area, c.d_scal, c.d_res, c.fin,
+                       c.fin_ticket, a);
 }
 
 void launch_trial(Ctx &c, Pair<const float2> v, Pair<const float2> u, Pair<const float2> g,
@@ -844,7 +853,7 @@ void launch_trial(Ctx &c, Pair<const float2> v, Pair<const float2> u, Pair<const
     const int blocks = grid_for(c.n_place, kBlock * kSortCellsPerThread, INT_MAX);
     hipLaunchKernelGGL(k_trial, dim3(blocks), dim3(kBlock), 0, s, v, u, g, c.wh, c.n_place,
                        geom_args(c), c.Mx, c.My, c.d_scal, v_new, u_new,
-                       count ? c.sort_count : nullptr);
+                       count ? c.sort_count : nullptr, count ? c.part + c.part_trial_off : nullptr);
 }
 
 void launch_step_begin(Ctx &c, const CtlParams &p, hipStream_t s)
