@@ -7,6 +7,10 @@
       FETCH_SIZE and WRITE_SIZE per dispatch (two --pmc passes) for the
       placement kernels, scaled by the same counters read on a device copy of
       known size -> profiles/traffic_<workload>_<prec>.md
+  python -m aieplace_amd.profiling mfma    [--workload stdcell_2m] [--precision 0]
+      matrix-core utilisation of the transform products (one --pmc pass of SQ
+      counters): SQ_VALU_MFMA_BUSY_CYCLES / (4 SIMDs x SQ_BUSY_CU_CYCLES) per
+      kernel, with the products' achieved TFLOP/s -> profiles/mfma_<workload>_<prec>.md
   python -m aieplace_amd.profiling drive ...
       the profiled program itself (run under rocprofv3 by the two commands
       above): a full placement, then `reps` ep_profile repetitions of each
@@ -254,6 +258,74 @@ def traffic(args) -> str:
     return path
 
 
+# ---------------------------------------------------------------- MFMA utilisation
+
+MFMA_COUNTERS = ["SQ_VALU_MFMA_BUSY_CYCLES", "SQ_BUSY_CU_CYCLES", "SQ_INSTS_MFMA", "SQ_INSTS_VALU",
+                 "SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_VALU"]
+PEAK_TFLOPS = {0: 157.3, 1: 2500.0, 2: 2500.0}  # dense MFMA: FP32, BF16, FP16
+
+
+def mfma(args) -> str:
+    prec = PREC_NAME[args.precision]
+    od = os.path.join(args.scratch, f"pmc_mfma_{args.workload}_{args.precision}")
+    full, _ = _rocprof(["--pmc"] + MFMA_COUNTERS, _drive_cmd(args), od, args.timeout)
+    rows = _csv(od, "counter_collection.csv")
+    per = {c: _per_kernel(_dispatches(rows, c), args.reps) for c in MFMA_COUNTERS}
+    # timings from the same driver run without the profiler (counter collection
+    # serialises the dispatches)
+    plain = subprocess.run(_drive_cmd(args), cwd=ROOT, capture_output=True, text=True, timeout=args.timeout)
+    drv = None
+    for line in plain.stdout.splitlines():
+        if line.startswith('{"run"'):
+            drv = json.loads(line)
+    if plain.returncode != 0 or drv is None:
+        raise RuntimeError(f"driver run failed ({plain.returncode}):\n{plain.stderr[-2000:]}")
+    prof = {q["name"]: q for q in drv["profile"]}
+    lines = [f"# Matrix-core utilisation: {args.workload}, {prec} transforms", "",
+             "One rocprofv3 --pmc pass of SQ counters over the driver (a full placement, then "
+             f"{args.reps} ep_profile repetitions of each item; per-dispatch values are the mean of each "
+             "kernel's last dispatches).  MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / (4 SIMDs x "
+             "SQ_BUSY_CU_CYCLES): the share of SIMD cycles, while the CUs are busy, in which the matrix core "
+             "works.  VALU active and waiting are per wave: SQ_ACTIVE_INST_VALU (quad-cycles, 4 cycles "
+             "each) and SQ_WAIT_INST_ANY over SQ_WAVE_CYCLES.", "",
+             f"Written by `python -m aieplace_amd.profiling mfma --workload {args.workload} --precision "
+             f"{args.precision}`, which runs (from the repository root):", "", "```",
+             " ".join(full).replace(sys.executable, "python3"), "```", "",
+             "| kernel | waves | MFMA instr / wave | MFMA busy | VALU instr / wave | VALU active | waiting |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    busy_m = busy_c = 0.0
+    for k in sorted(per["SQ_WAVES"]):
+        w = per["SQ_WAVES"][k]
+        if w <= 0:
+            continue
+        mb, cb = per["SQ_VALU_MFMA_BUSY_CYCLES"].get(k, 0.0), per["SQ_BUSY_CU_CYCLES"].get(k, 0.0)
+        wc = per["SQ_WAVE_CYCLES"].get(k, 0.0)
+        util = mb / (4.0 * cb) if cb else 0.0
+        va = per["SQ_ACTIVE_INST_VALU"].get(k, 0.0) / wc if wc else 0.0
+        wt = per["SQ_WAIT_INST_ANY"].get(k, 0.0) / wc if wc else 0.0
+        lines.append(f"| {k} | {w:.0f} | {per['SQ_INSTS_MFMA'].get(k, 0.0) / w:.1f} | {100 * util:.1f}% | "
+                     f"{per['SQ_INSTS_VALU'].get(k, 0.0) / w:.0f} | {100 * va:.1f}% | {100 * wt:.1f}% |")
+        if _family(k) == "field_transforms":
+            busy_m += mb
+            busy_c += cb
+    q = prof.get("field_transforms")
+    if q and busy_c:
+        tf = q["flops"] / (q["avg_us"] * 1e-6) / 1e12
+        lines += ["", f"Transform products together (the four dispatches of one field solve): MFMA busy "
+                      f"{100 * busy_m / (4 * busy_c):.1f}% of SIMD cycles; {q['avg_us']:.1f} us per solve "
+                      f"(ep_profile, HIP events, the driver run again without the profiler) for "
+                      f"{q['flops']:.4g} flops = {tf:.1f} TFLOP/s, "
+                      f"{100 * tf / PEAK_TFLOPS[args.precision]:.1f}% of the dense "
+                      f"{'FP32' if args.precision == 0 else '16-bit'} MFMA peak "
+                      f"({PEAK_TFLOPS[args.precision]:g} TFLOP/s; the split forms count each of the three "
+                      "products' flops).", ""]
+        lines.append(f"mfma field_transforms busy {busy_m / (4 * busy_c):.4f} tflops {tf:.2f}")
+    path = os.path.join(PROFILES, f"mfma_{args.workload}_{prec}.md")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
 def read_traffic(workload: str, precision: int):
     """{item: measured HBM bytes per call} from a committed traffic summary, or {}."""
     path = os.path.join(PROFILES, f"traffic_{workload}_{PREC_NAME[precision]}.md")
@@ -268,7 +340,7 @@ def read_traffic(workload: str, precision: int):
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m aieplace_amd.profiling")
-    ap.add_argument("cmd", choices=["drive", "stats", "traffic"])
+    ap.add_argument("cmd", choices=["drive", "stats", "traffic", "mfma"])
     ap.add_argument("--workload", default="stdcell_2m")
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--reps", type=int, default=5)
@@ -282,7 +354,7 @@ def main(argv=None) -> int:
         return 0
     os.makedirs(PROFILES, exist_ok=True)
     t0 = time.time()
-    path = stats(args) if args.cmd == "stats" else traffic(args)
+    path = {"stats": stats, "traffic": traffic, "mfma": mfma}[args.cmd](args)
     print(f"wrote {path} in {time.time() - t0:.1f}s", flush=True)
     return 0
 
