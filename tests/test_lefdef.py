@@ -343,3 +343,41 @@ def test_errors(tmp_path):
     with pytest.raises(api.EplaceError) as e:
         LefDef([lef], str(tmp_path / "nodie.def"))
     assert e.value.code == api.EP_EIO
+
+
+def test_mutated_inputs_never_crash(tmp_path):
+    """Truncated, token-dropped and token-duplicated LEF/DEF text: the reader
+    returns OK or an error status, and a design it accepts has a consistent
+    netlist view (the native parser must not crash or read out of bounds)."""
+    rng = np.random.default_rng(7)
+    base_def = open(os.path.join(GOLD, "simple.def")).read().replace("u4[0]", "u4")
+    texts = [(SIMPLE_LEF, base_def), (SIMPLE_LEF, ROBUST_DEF)]
+    codes = set()
+    for trial in range(300):
+        lef, df = texts[trial % 2]
+        which = rng.integers(0, 2)
+        tok = (lef if which == 0 else df).split(" ")
+        op = rng.integers(0, 3)
+        if op == 0:
+            tok = tok[:rng.integers(0, len(tok))]
+        elif op == 1:
+            for _ in range(rng.integers(1, 6)):
+                if tok:
+                    del tok[rng.integers(0, len(tok))]
+        else:
+            for _ in range(rng.integers(1, 6)):
+                k = rng.integers(0, len(tok))
+                tok.insert(k, tok[rng.integers(0, len(tok))])
+        text = " ".join(tok)
+        lef_t, def_t = (text, df) if which == 0 else (lef, text)
+        try:
+            with read(tmp_path, lef_t, def_t) as d:
+                nl = d.netlist()
+                n_all = nl.n_mov + nl.n_fix
+                assert len(nl.w) == n_all and (nl.pin_cell < max(n_all, 1)).all()
+                assert nl.net_start[0] == 0 and nl.net_start[-1] == nl.n_pins
+            codes.add(0)
+        except api.EplaceError as e:
+            assert e.code in (api.EP_EIO, api.EP_EINVAL), str(e)
+            codes.add(e.code)
+    assert 0 in codes and len(codes) >= 2
