@@ -362,11 +362,37 @@ __device__ inline void wa_net(int s, int e, bool active, int lane, float ginv,
 // off[key] = first slot of key, counts cleared, and the density block table
 // dtab[s] = first block of super tile s (level `level`, K cells per block),
 // dtab[NS] = total blocks.
+// Inclusive prefix sum of one int per thread over a kSortScanThreads block:
+// shuffles within each wave, then the wave totals (two block barriers).
+__device__ inline int block_inclusive_scan(int x)
+{
+    constexpr int kWaves = kSortScanThreads / 64;
+    __shared__ int wsum[kWaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int t = lane < kWaves ? wsum[lane] : 0;
+#pragma unroll
+        for (int d = 1; d < kWaves; d <<= 1) {
+            const int y = __shfl_up(t, d, 64);
+            if (lane >= d) t += y;
+        }
+        if (lane < kWaves) wsum[lane] = t;
+    }
+    __syncthreads();
+    return w > 0 ? x + wsum[w - 1] : x;
+}
+
 __device__ inline void sort_scan_block(int32_t *__restrict__ count, int32_t *__restrict__ off,
                                        int32_t *__restrict__ dtab, int level, int K)
 {
     constexpr int per = kSortTiles / kSortScanThreads;
-    __shared__ int part[kSortScanThreads];
     __shared__ int so[kSortTiles + 1];
     const int t = threadIdx.x;
     int v[per], s = 0;
@@ -375,15 +401,7 @@ __device__ inline void sort_scan_block(int32_t *__restrict__ count, int32_t *__r
         v[k] = count[t * per + k];
         s += v[k];
     }
-    part[t] = s;
-    __syncthreads();
-    for (int d = 1; d < kSortScanThreads; d <<= 1) {  // Hillis-Steele inclusive scan
-        const int add = t >= d ? part[t - d] : 0;
-        __syncthreads();
-        part[t] += add;
-        __syncthreads();
-    }
-    int run = part[t] - s;
+    int run = block_inclusive_scan(s) - s;
 #pragma unroll
     for (int k = 0; k < per; ++k) {
         off[t * per + k] = run;
@@ -406,16 +424,8 @@ __device__ inline void sort_scan_block(int32_t *__restrict__ count, int32_t *__r
         }
         sb += nb[k];
     }
-    __syncthreads();
-    part[t] = sb;
-    __syncthreads();
-    for (int d = 1; d < kSortScanThreads; d <<= 1) {
-        const int add = t >= d ? part[t - d] : 0;
-        __syncthreads();
-        part[t] += add;
-        __syncthreads();
-    }
-    run = part[t] - sb;
+    __syncthreads();  // the wave totals of the first scan are read before the second reuses them
+    run = block_inclusive_scan(sb) - sb;
 #pragma unroll
     for (int k = 0; k < per; ++k) {
         const int st = t * per + k;
