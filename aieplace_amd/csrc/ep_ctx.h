@@ -36,7 +36,7 @@ constexpr int kBlock = 256;
 constexpr int kWaGridCap = 16384;
 constexpr int kCellGridCap = 16384;
 // Spatial counting sort of cells: kSortSide x kSortSide coarse tiles.
-constexpr int kSortSide = 64;
+constexpr int kSortSide = 32;
 constexpr int kSortTiles = kSortSide * kSortSide;
 constexpr int kSortCellsPerThread = 16;
 // After a rejected model proposal the model sits out 2, 4, ... up to this many

@@ -349,7 +349,7 @@ __device__ inline void wa_net(int s, int e, bool active, int lane, float ginv,
 // ------------------------------------------------------------------ density scatter
 //
 // Cells are counting-sorted by the coarse tile of their smoothed rectangle's
-// lower-left bin (64 x 64 coarse tiles in Morton order).  A super tile of level
+// lower-left bin (kSortSide x kSortSide coarse tiles in Morton order).  A super tile of level
 // k is an aligned block of 2^k x 2^k coarse tiles, i.e. 4^k consecutive keys,
 // so its cells are one contiguous run of the sorted order.  Every cell of a
 // super tile starts inside the super tile's bins, so its bins lie in the

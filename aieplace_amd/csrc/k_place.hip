@@ -41,7 +41,7 @@ __global__ void k_fill_maps(unsigned long long *__restrict__ acc_field,
     }
 }
 
-// ---- spatial counting sort of the cells by coarse tile (64 x 64 tiles in
+// ---- spatial counting sort of the cells by coarse tile (kSortSide^2 tiles in
 // Morton order).  Any order gives the same maps; a spatially sorted order makes
 // each density block's cells share a small bin window, so the block can
 // accumulate in LDS and flush with coalesced atomics.
