@@ -134,3 +134,24 @@ def test_global_placement_then_legalisation():
         f"(+{100 * (h_legal / h_global - 1):.2f}%), mean disp {info['total_disp'] / nl.n_mov:.3f}, "
         f"max {info['max_disp']:.2f}, {info['seconds'] * 1e3:.1f} ms")
     assert h_legal < 1.2 * h_global
+
+
+def test_command_line_run(tmp_path):
+    """eplace_run on a LEF/DEF directory: places, legalises and writes the DEF,
+    which reads back legal."""
+    import subprocess
+    from support import check_legal
+    say("case eplace_run mesh 576 cells")
+    gen = A.generate("mesh", seed=1)
+    lef, df = netlist_lefdef(gen, name="mesh576")
+    (tmp_path / "cells.lef").write_text(lef)
+    (tmp_path / "floorplan.def").write_text(df)
+    runner = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "aieplace_amd", "eplace_run")
+    r = subprocess.run([runner, str(tmp_path), "--bins", "32", "--density", "1.0", "--legalize",
+                        "--out", str(tmp_path / "out")], capture_output=True, text=True, timeout=120)
+    say(r.stdout[-600:])
+    assert r.returncode == 0, r.stderr
+    assert "final HPWL" in r.stdout and "legalised" in r.stdout
+    with LefDef([str(tmp_path / "cells.lef")], str(tmp_path / "out" / "mesh576.def")) as d:
+        back = d.netlist()
+    check_legal(back, back.x[:back.n_mov], back.y[:back.n_mov], 1.0, 1.0)

@@ -381,3 +381,21 @@ def test_mutated_inputs_never_crash(tmp_path):
             assert e.code in (api.EP_EIO, api.EP_EINVAL), str(e)
             codes.add(e.code)
     assert 0 in codes and len(codes) >= 2
+
+
+RUNNER = os.path.join(os.path.dirname(HERE), "aieplace_amd", "eplace_run")
+
+
+def test_command_line_errors(tmp_path):
+    """eplace_run (the AIEplace.exe counterpart): bad arguments exit 2, an
+    unreadable design exits 1 with the library's message (no GPU is touched)."""
+    import subprocess
+    r = subprocess.run([RUNNER], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([RUNNER, str(tmp_path), "--precision", "fp8"], capture_output=True, text=True)
+    assert r.returncode == 2
+    r = subprocess.run([RUNNER, str(tmp_path / "missing")], capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot open" in r.stderr
+    (tmp_path / "cells.lef").write_text(SIMPLE_LEF)
+    r = subprocess.run([RUNNER, str(tmp_path)], capture_output=True, text=True)  # no .def
+    assert r.returncode == 1 and "floorplan.def" in r.stderr
