@@ -532,7 +532,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     }
     const size_t nb = (size_t)c->Mx * c->My;
     c->ovf_blocks = (int)std::min<size_t>(1024, (nb + kBlock * 4 - 1) / (kBlock * 4));
-    c->cell_blocks = (int)std::min<int64_t>(kCellGridCap, ((int64_t)c->n_place + kBlock - 1) / kBlock);
+    c->cell_blocks = (int)std::min<int64_t>(kCellGridCap, ((int64_t)c->n_place + 2 * kBlock - 1) / (2 * kBlock));
     c->part_wa_off = 0;
     c->part_ovf_off = 2 * c->wa_part_blocks;
     c->part_cell_off = c->part_ovf_off + 4 * c->ovf_blocks;

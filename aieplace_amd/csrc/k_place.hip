@@ -324,8 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
     const float lambda = sc->lambda;
     // sums: dv2, dg2, |gw|, |f|, g2, nonfinite ; max: |g|
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; c < n_place;
-         c += gridDim.x * kBlock) {
+    auto one = [&](int c) {
         const float2 p = pos[c], s = wh[c];
         const float2 v0 = vref ? vref[c] : p;  // independent loads first
         const float2 g0 = gref ? gref[c] : make_float2(0.f, 0.f);
@@ -357,6 +356,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
         acc[4] += (double)gx * gx + (double)gy * gy;
         if (!isfinite(gx) || !isfinite(gy)) acc[5] += 1.0;
         acc[6] = fmax(acc[6], fmax(fabs((double)gx), fabs((double)gy)));
+    };
+    // two cells per thread and step, so two gather chains are in flight
+    const int stride = gridDim.x * kBlock;
+    for (int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; c < n_place; c += 2 * stride) {
+        one(c);
+        if (c + stride < n_place) one(c + stride);
     }
     block_reduce<7, 1>(acc, part + 7 * blockIdx.x);
 }
