@@ -11,6 +11,9 @@
       matrix-core utilisation of the transform products (one --pmc pass of SQ
       counters): SQ_VALU_MFMA_BUSY_CYCLES / (4 SIMDs x SQ_BUSY_CU_CYCLES) per
       kernel, with the products' achieved TFLOP/s -> profiles/mfma_<workload>_<prec>.md
+  python -m aieplace_amd.profiling batch   [--instances 64] [--precision 0]
+      kernel-trace statistics of the batched engine on the 64-instance
+      workload (two full runs: warm-up and measured) -> profiles/kernels_batch<n>_<prec>.md
   python -m aieplace_amd.profiling drive ...
       the profiled program itself (run under rocprofv3 by the two commands
       above): a full placement, then `reps` ep_profile repetitions of each
@@ -82,6 +85,23 @@ def drive(args) -> None:
                       "profile": prof}), flush=True)
 
 
+def drive_batch(args) -> None:
+    """The batched engine on `instances` generated instances: a warm-up run
+    (graph capture, tile autotuning) and a measured run."""
+    sys.path.insert(0, ROOT)
+    import aieplace_amd as A
+    from aieplace_amd.batch import BatchPlacer, batch_workload
+    nls, bins = batch_workload(args.instances)
+    p = A.placer_params(A.CONFIGS["stdcell_200k"], precision=args.precision)
+    with BatchPlacer(nls, bins, p) as bp:
+        bp.run()
+        bp.reset()
+        r = bp.run()
+    run = {k: float(r[k]) for k in ("instance_iterations", "seconds", "evals", "launches", "finished")}
+    run["movable_cells"] = int(sum(nl.n_mov for nl in nls))
+    print(json.dumps({"run": run}), flush=True)
+
+
 def _drive_cmd(args, extra=()):
     cmd = [sys.executable, "-u", "-m", "aieplace_amd.profiling", "drive", "--workload", args.workload,
            "--precision", str(args.precision), "--reps", str(args.reps)]
@@ -147,6 +167,37 @@ def stats(args) -> str:
         for q in res["profile"]:
             lines.append(f"| {q['name']} | {q['launches']} | {q['avg_us']:.2f} | {q['bytes']:.4g} | {q['flops']:.4g} |")
     path = os.path.join(PROFILES, f"kernels_{args.workload}_{prec}.md")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
+def batch_stats(args) -> str:
+    out_dir = os.path.join(args.scratch, f"stats_batch{args.instances}_{args.precision}")
+    cmd = [sys.executable, "-u", "-m", "aieplace_amd.profiling", "drive-batch", "--precision",
+           str(args.precision), "--instances", str(args.instances)]
+    full, res = _rocprof(["--kernel-trace", "--stats"], cmd, out_dir, args.timeout)
+    rows = _csv(out_dir, "kernel_stats.csv")
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    prec = PREC_NAME[args.precision]
+    lines = [f"# Kernel statistics: batched engine, {args.instances} instances, {prec} transforms", "",
+             "Command (from the repository root):", "", "```", " ".join(full).replace(sys.executable, "python3"),
+             "```", "",
+             "Two full batched runs (the first captures the graphs and autotunes the tiles; the second is the "
+             "measured one), so every count is about twice one run's.  The WA gradient runs on a second captured "
+             "branch while the running instances hold >= 400k cells, so the shares add up to more than the "
+             "wall time.", ""]
+    if res:
+        r = res["run"]
+        lines += [f"Measured run (under the profiler): {r['movable_cells']} movable cells, "
+                  f"{r['instance_iterations']:.0f} instance-iterations, {r['evals']:.0f} evaluations, "
+                  f"{r['launches']:.0f} graph replays, {r['finished']:.0f} instances finished, "
+                  f"{r['seconds']:.3f} s ({r['instance_iterations'] / r['seconds']:.0f} instance-iterations/s).", ""]
+    lines += ["| kernel | calls | avg us | total ms | share |", "|---|---:|---:|---:|---:|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        lines.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.2f} | {100 * float(r['TotalDurationNs']) / tot:.1f}% |")
+    path = os.path.join(PROFILES, f"kernels_batch{args.instances}_{prec}.md")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     return path
@@ -340,10 +391,11 @@ def read_traffic(workload: str, precision: int):
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m aieplace_amd.profiling")
-    ap.add_argument("cmd", choices=["drive", "stats", "traffic", "mfma"])
+    ap.add_argument("cmd", choices=["drive", "drive-batch", "stats", "traffic", "mfma", "batch"])
     ap.add_argument("--workload", default="stdcell_2m")
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--instances", type=int, default=64, help="batch: instances of the batched workload")
     ap.add_argument("--copy-mib", type=int, default=0, help="drive: copy calibration instead of placement")
     ap.add_argument("--calib-mib", type=int, default=256, help="traffic: size of the calibration copy")
     ap.add_argument("--scratch", default=os.path.join(ROOT, "gpurun_out", "rocprof"))
@@ -352,9 +404,12 @@ def main(argv=None) -> int:
     if args.cmd == "drive":
         drive(args)
         return 0
+    if args.cmd == "drive-batch":
+        drive_batch(args)
+        return 0
     os.makedirs(PROFILES, exist_ok=True)
     t0 = time.time()
-    path = {"stats": stats, "traffic": traffic, "mfma": mfma}[args.cmd](args)
+    path = {"stats": stats, "traffic": traffic, "mfma": mfma, "batch": batch_stats}[args.cmd](args)
     print(f"wrote {path} in {time.time() - t0:.1f}s", flush=True)
     return 0
 
