@@ -756,7 +756,7 @@ int eo_place(const eo_design *d, const eo_params *p, double *x, double *y, eo_it
     for (it = 0; it < p->max_iters; ++it) {
         double ak1 = 0.5 * (1.0 + sqrt(4.0 * ak * ak + 1.0));
         double coef = (ak - 1.0) / ak1;
-        double alpha_hat = alpha;
+        double alpha_hat = alpha, alpha_used = alpha;
         int trials = 0;
         for (int t = 0; t < p->max_backtrack; ++t) {
             for (int c = 0; c < np; ++c) {
@@ -776,6 +776,7 @@ int eo_place(const eo_design *d, const eo_params *p, double *x, double *y, eo_it
             double nn = sqrt(dist2(np, nvx, nvy, vx, vy));
             double dd = sqrt(dist2(np, ngx, ngy, gx, gy));
             alpha_hat = dd > 0 ? nn / dd : alpha;
+            alpha_used = alpha;
             if (alpha_hat >= 0.95 * alpha || t == p->max_backtrack - 1) break;
             alpha = alpha_hat;
         }
@@ -793,6 +794,8 @@ int eo_place(const eo_design *d, const eo_params *p, double *x, double *y, eo_it
             log[it].lambda = lambda;
             log[it].gamma = gamma;
             log[it].alpha = alpha;
+            log[it].alpha_used = alpha_used;
+            log[it].coef = coef;
         }
         /* lambda / gamma schedules (RePlAce-style mu, DREAMplace gamma) */
         double dh = cur_hpwl - prev_hpwl;

@@ -147,7 +147,12 @@ typedef struct eo_params {
 typedef struct eo_iter_log {
     int32_t iter;
     int32_t evals;     /* gradient evaluations in this iteration       */
-    double hpwl, overflow, lambda, gamma, alpha;
+    double hpwl, overflow;   /* at the accepted point v_k               */
+    double lambda, gamma;    /* used by this iteration's trials          */
+    double alpha;            /* alpha_hat of the accepted trial: the next
+                                iteration's first step                   */
+    double alpha_used;       /* step length of the accepted trial        */
+    double coef;             /* Nesterov coefficient (a_k - 1)/a_{k+1}   */
 } eo_iter_log;
 
 /* Runs ePlace Nesterov with backtracking from (x,y) (cells [mov|fill]) until

@@ -58,7 +58,7 @@ class EoIterLog(C.Structure):
     _fields_ = [
         ("iter", C.c_int32), ("evals", C.c_int32), ("hpwl", C.c_double),
         ("overflow", C.c_double), ("lambda_", C.c_double), ("gamma", C.c_double),
-        ("alpha", C.c_double),
+        ("alpha", C.c_double), ("alpha_used", C.c_double), ("coef", C.c_double),
     ]
 
 
@@ -321,7 +321,8 @@ class Design:
         it = lib().eo_place(self.ptr, C.byref(p), x, y, C.addressof(logs), log_cap,
                             C.byref(fh), C.byref(fo), C.byref(ev))
         log = [dict(iter=l.iter, evals=l.evals, hpwl=l.hpwl, overflow=l.overflow,
-                    lam=l.lambda_, gamma=l.gamma, alpha=l.alpha) for l in logs[:min(it, log_cap)]]
+                    lam=l.lambda_, gamma=l.gamma, alpha=l.alpha, alpha_used=l.alpha_used,
+                    coef=l.coef) for l in logs[:min(it, log_cap)]]
         return x, y, dict(iters=it, hpwl=fh.value, overflow=fo.value, evals=ev.value, log=log)
 
 
