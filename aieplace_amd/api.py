@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libeplace.so")
 EP_OK, EP_EINVAL, EP_EDEVICE, EP_ENAN, EP_ENOMEM, EP_ESTATE, EP_ENOTSUP, EP_EIO = range(8)
 EP_PREC_FP32, EP_PREC_BF16X3, EP_PREC_FP16X3 = 0, 1, 2
 EP_GEN_MESH, EP_GEN_STDCELL = 0, 1
+EP_TR_NONE, EP_TR_DCT, EP_TR_IDCT, EP_TR_IDXST = -1, 0, 1, 2
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
@@ -41,6 +42,7 @@ class EpParams(C.Structure):
         ("max_backtrack", C.c_int32), ("stop_overflow", C.c_float), ("max_iters", C.c_int32),
         ("init_disp_bins", C.c_float), ("use_fillers", C.c_int32), ("filler_seed", C.c_uint64),
         ("precision", C.c_int32), ("use_graph", C.c_int32), ("learned_step", C.c_int32),
+        ("fork_min_cells", C.c_int32),
     ]
 
 
@@ -86,6 +88,7 @@ EXPORTS = [
     "ep_get_wa_grad", "ep_density", "ep_get_density", "ep_field", "ep_get_field", "ep_field_error", "ep_gradient",
     "ep_cell_bins", "ep_hpwl", "ep_overflow", "ep_init_optimizer", "ep_step", "ep_run",
     "ep_set_precision", "ep_profile", "ep_take_step_records", "ep_set_step_model", "ep_wa_packet",
+    "ep_transform",
     "ep_gen_default_params", "ep_gen_create", "ep_gen_free",
     "ep_mlp_default_train_params", "ep_mlp_train", "ep_mlp_forward", "ep_mlp_grad",
     "ep_batch_create", "ep_batch_destroy", "ep_batch_reset", "ep_batch_init", "ep_batch_run",
@@ -169,6 +172,7 @@ def _load():
     L.ep_take_step_records.argtypes = [vp, _fp, _fp, C.c_int32, _ip]
     L.ep_set_step_model.argtypes = [vp, _fp, C.c_int32]
     L.ep_wa_packet.argtypes = [_fp, C.c_int32, C.c_float, _fp, C.c_int32]
+    L.ep_transform.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp, C.c_int32]
     L.ep_gen_default_params.argtypes = [C.POINTER(EpGenParams), C.c_int32]
     L.ep_gen_default_params.restype = None
     L.ep_gen_create.argtypes = [C.POINTER(EpGenParams), C.POINTER(C.POINTER(EpDesign))]

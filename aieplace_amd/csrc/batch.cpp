@@ -625,22 +625,36 @@ extern "C" ep_status ep_batch_init(ep_batch *b)
     return EP_OK;
 }
 
+// The graph holds device work only (kernels, the fork/join events): the host
+// reads the control state with its own copy after each replay.
 static ep_status build_graph(ep_batch *b)
 {
     const BDev d = b->dev();
     hipGraph_t g = nullptr;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
     EPB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int t = 0; t < kTrialsPerLaunch; ++t) {
+    hipError_t e = hipSuccess;
+    for (int t = 0; t < kTrialsPerLaunch && e == hipSuccess; ++t) {
         bl_move(d, kMoveTrial, b->sort_blocks, b->stream);
-        if (b->active_cells() >= kForkMinCells) EPB_HIP(b->eval_forked(d, kAtTrial, true, b->stream));
+        if (b->active_cells() >= fork_min_cells(b->prm)) e = b->eval_forked(d, kAtTrial, true, b->stream);
         else b->eval(d, kAtTrial, true, b->stream);
         bl_control(d, kCtlIterate, b->stream);
     }
-    (void)hipMemcpyAsync(b->h_st, b->d_st, sizeof(BState) * b->n, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {  // leave capture mode before reporting
+        (void)hipStreamEndCapture(b->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return fail(EP_EDEVICE, "ep_batch: graph capture failed: %s", hipGetErrorString(e));
+    }
     EPB_HIP(hipStreamEndCapture(b->stream, &g));
-    hipError_t e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(EP_EDEVICE, "ep_batch: hipGraphInstantiate: %s", hipGetErrorString(e));
+    if (e != hipSuccess) {
+        b->exec = nullptr;
+        return fail(EP_EDEVICE, "ep_batch: hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
     return EP_OK;
 }
 
@@ -661,6 +675,7 @@ extern "C" ep_status ep_batch_run(ep_batch *b, int32_t max_launches, ep_batch_ru
     auto count = [](const std::vector<char> &a) { return (int)std::count(a.begin(), a.end(), 1); };
     while (launches < max_launches && !all_done()) {
         EPB_HIP(hipGraphLaunch(b->exec, b->stream));
+        EPB_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(BState) * b->n, hipMemcpyDeviceToHost, b->stream));
         EPB_HIP(hipStreamSynchronize(b->stream));
         ++launches;
         if ((st = check_errors(b)) != EP_OK) return st;

@@ -103,6 +103,27 @@ void enqueue_eval(Ctx &c, const float2 *pos, const float2 *vref, const float2 *g
     (void)hipMemcpyAsync(c.h_res, c.d_res, sizeof(DevResult), hipMemcpyDeviceToHost, s);
 }
 
+// Leaves stream capture after an error between hipStreamBeginCapture and
+// hipStreamEndCapture (the stream would otherwise stay in capture mode and fail
+// every later call); the partial graph is discarded.
+void abort_capture(hipStream_t s)
+{
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+}
+
+#define EP_CAP(call, stream)                                                                 \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            abort_capture(stream);                                                           \
+            return fail(EP_EDEVICE, "%s failed during capture: %s (%s:%d)", #call,           \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                          \
+        }                                                                                    \
+    } while (0)
+
 ep_status sync(Ctx &c)
 {
     EP_HIP(hipStreamSynchronize(c.stream));
@@ -133,6 +154,7 @@ ep_status trial_eval(Ctx &c)
             launch_mlp_propose(c, c.stream);
             launch_trial(c, c.V[a], c.U[a], c.G[a], c.V[b], c.U[b], true, c.stream);
             enqueue_eval(c, c.V[b], c.V[a], c.G[a], c.G[b], nullptr, nullptr, c.stream, true);
+            EP_CAP(hipGetLastError(), c.stream);
             EP_HIP(hipStreamEndCapture(c.stream, &g));
             hipError_t e = hipGraphInstantiate(&c.graph_exec[a], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -180,6 +202,7 @@ extern "C" void ep_default_params(ep_params *p)
     p->precision = EP_PREC_FP32;
     p->use_graph = 0;
     p->learned_step = 0;
+    p->fork_min_cells = 0;
 }
 
 static ep_status check_precision(int32_t precision, const char *who)
@@ -207,6 +230,7 @@ ep_status ep::validate_netlist(const ep_netlist *nl, const ep_params *p)
         return fail(EP_EINVAL, "ep_create: target_density must be in (0, 1]");
     if (!(p->gamma_bins > 0.f)) return fail(EP_EINVAL, "ep_create: gamma_bins must be > 0");
     if (p->max_backtrack < 1) return fail(EP_EINVAL, "ep_create: max_backtrack must be >= 1");
+    if (p->fork_min_cells < 0) return fail(EP_EINVAL, "ep_create: fork_min_cells must be >= 0");
     ep_status pst = check_precision(p->precision, "ep_create");
     if (pst != EP_OK) return pst;
     const int n_all = nl->n_mov + nl->n_fix;
@@ -946,6 +970,12 @@ extern "C" ep_status ep_init_optimizer(ep_ctx *c)
     c->h_dg = den;
     c->h_sum_ratio = sf > 0 ? sw / sf : 1.0;
     c->h_evals = 1;
+    // the step model's backoff and counters belong to one run: a new run (after
+    // ep_set_positions or a fresh context) starts with the model proposing
+    c->model_skip = 0;
+    c->model_backoff = 0;
+    c->learned_proposals = 0;
+    c->learned_accepted = 0;
     c->opt_ready = true;
     return EP_OK;
 }
@@ -1152,18 +1182,18 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
         // (k_finalize) or by k_step_begin
         launch_trial(x, vc, uc, gc, vn, un, true, s);
         launch_sort(x, vt, true, s);
-        if (x.n_place >= kForkMinCells) {
+        if (x.n_place >= fork_min_cells(x.prm)) {
             // fork: the WA gradient and the per-cell pin sums beside the density
             // scatter and the transform products
-            EP_HIP(hipEventRecord(x.fork, s));
-            EP_HIP(hipStreamWaitEvent(x.side, x.fork, 0));
+            EP_CAP(hipEventRecord(x.fork, s), s);
+            EP_CAP(hipStreamWaitEvent(x.side, x.fork, 0), s);
             launch_wa(x, vt, x.side);
             launch_pin_sum(x, x.side);
-            EP_HIP(hipEventRecord(x.join, x.side));
+            EP_CAP(hipEventRecord(x.join, x.side), s);
             launch_density(x, s);
             launch_overflow(x, s);
             launch_field(x, x.prm.precision, s);
-            EP_HIP(hipStreamWaitEvent(s, x.join, 0));
+            EP_CAP(hipStreamWaitEvent(s, x.join, 0), s);
             launch_cell_grad(x, vt, none, gc, gn, nullptr, nullptr, s, x.gw_cell);
         } else {
             launch_density(x, s);
@@ -1174,6 +1204,7 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
         }
         launch_finalize_decide(x, p, s);
     }
+    EP_CAP(hipGetLastError(), s);
     EP_HIP(hipStreamEndCapture(s, &g));
     hipError_t e = hipGraphInstantiate(&c->ctl_exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1266,8 +1297,11 @@ extern "C" ep_status ep_run(ep_ctx *c, ep_run_info *info, ep_step_info *log, int
     if (!c) return fail(EP_EINVAL, "ep_run: null context");
     auto t0 = std::chrono::steady_clock::now();
     ep_status st;
+    // counts of this call only (the optimiser set-up's evaluations included when
+    // this call made it)
     const int64_t ev0 = c->evals;
     if (!c->opt_ready && (st = ep_init_optimizer(c)) != EP_OK) return st;
+    const int32_t lp0 = c->learned_proposals, la0 = c->learned_accepted;
     if (c->prm.use_graph && c->iteration < c->prm.max_iters) {
         int n = 0;
         if ((st = run_device(c, log, log_cap, &n)) != EP_OK) return st;
@@ -1285,8 +1319,8 @@ extern "C" ep_status ep_run(ep_ctx *c, ep_run_info *info, ep_step_info *log, int
     if (info) {
         info->iterations = c->iteration;
         info->evals = c->evals - ev0;
-        info->learned_proposals = c->learned_proposals;
-        info->learned_accepted = c->learned_accepted;
+        info->learned_proposals = c->learned_proposals - lp0;
+        info->learned_accepted = c->learned_accepted - la0;
         info->hpwl = c->last_hpwl;
         info->overflow = c->last_tau;
         info->seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -1420,24 +1454,37 @@ extern "C" ep_status ep_set_step_model(ep_ctx *c, const float *w, int32_t n)
     return EP_OK;
 }
 
-extern "C" ep_status ep_wa_packet(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out)
+// Checks an AIE-format packet (see ep_wa_packet) and returns its geometry.
+static ep_status check_packet(const float *in, int32_t n_in, float gamma, const float *out, int32_t n_out,
+                              const char *who, int *net_size, int *n_nets, int *groups, int64_t *n_data)
 {
-    if (!in || !out || n_in < 4) return fail(EP_EINVAL, "ep_wa_packet: bad argument");
-    if (!(gamma > 0.f)) return fail(EP_EINVAL, "ep_wa_packet: gamma must be > 0");
+    if (!in || !out || n_in < 4) return fail(EP_EINVAL, "%s: bad argument", who);
+    if (!(gamma > 0.f)) return fail(EP_EINVAL, "%s: gamma must be > 0", who);
     const float fs = in[0], fn = in[1];
     if (!(fs >= 1.f) || fs != std::floor(fs) || !(fn >= 0.f) || fn != std::floor(fn) || fs > 65536.f ||
         fn > 1e8f)
-        return fail(EP_EINVAL, "ep_wa_packet: bad control words [%g, %g]", fs, fn);
-    const int net_size = (int)fs, n_nets = (int)fn;
-    const int groups = (n_nets + 7) / 8;
-    const int64_t n_data = (int64_t)groups * net_size * 8;
-    if (n_in < 4 + n_data) return fail(EP_EINVAL, "ep_wa_packet: packet holds %d values, needs %lld", n_in - 4, (long long)n_data);
-    if (n_out < n_data) return fail(EP_EINVAL, "ep_wa_packet: output holds %d values, needs %lld", n_out, (long long)n_data);
+        return fail(EP_EINVAL, "%s: bad control words [%g, %g]", who, fs, fn);
+    *net_size = (int)fs;
+    *n_nets = (int)fn;
+    *groups = (*n_nets + 7) / 8;
+    *n_data = (int64_t)*groups * *net_size * 8;
+    if (n_in < 4 + *n_data)
+        return fail(EP_EINVAL, "%s: packet holds %d values, needs %lld", who, n_in - 4, (long long)*n_data);
+    if (n_out < *n_data)
+        return fail(EP_EINVAL, "%s: output holds %d values, needs %lld", who, n_out, (long long)*n_data);
     int nd = 0;
-    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) return fail(EP_EDEVICE, "ep_wa_packet: no HIP device");
-    if (n_data == 0) return EP_OK;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) return fail(EP_EDEVICE, "%s: no HIP device", who);
+    return EP_OK;
+}
+
+extern "C" ep_status ep_wa_packet(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out)
+{
+    int net_size = 0, n_nets = 0, groups = 0;
+    int64_t n_data = 0;
+    ep_status st = check_packet(in, n_in, gamma, out, n_out, "ep_wa_packet", &net_size, &n_nets, &groups, &n_data);
+    if (st != EP_OK || n_data == 0) return st;
     float *d_in = nullptr, *d_out = nullptr;
-    ep_status st = dalloc(&d_in, (size_t)n_data);
+    st = dalloc(&d_in, (size_t)n_data);
     if (st == EP_OK) st = dalloc(&d_out, (size_t)n_data);
     if (st == EP_OK && hipMemcpy(d_in, in + 4, sizeof(float) * n_data, hipMemcpyHostToDevice) != hipSuccess)
         st = fail(EP_EDEVICE, "ep_wa_packet: upload failed");
@@ -1450,4 +1497,26 @@ extern "C" ep_status ep_wa_packet(const float *in, int32_t n_in, float gamma, fl
     if (d_in) (void)hipFree(d_in);
     if (d_out) (void)hipFree(d_out);
     return st;
+}
+
+extern "C" ep_status ep_transform(int32_t row_kind, int32_t col_kind, int32_t nx, int32_t ny, const float *in,
+                                  float *out, int32_t precision)
+{
+    if (!in || !out) return fail(EP_EINVAL, "ep_transform: null array");
+    auto kind_ok = [](int32_t k) { return k == EP_TR_NONE || k == EP_TR_DCT || k == EP_TR_IDCT || k == EP_TR_IDXST; };
+    if (!kind_ok(row_kind) || !kind_ok(col_kind))
+        return fail(EP_EINVAL, "ep_transform: unknown transform kind (%d, %d)", row_kind, col_kind);
+    if (row_kind == EP_TR_NONE && col_kind == EP_TR_NONE)
+        return fail(EP_EINVAL, "ep_transform: no axis to transform");
+    if (nx < 16 || ny < 16 || nx % 16 || ny % 16 || nx > 16384 || ny > 16384 || (int64_t)nx * ny > ((int64_t)1 << 26))
+        return fail(EP_EINVAL, "ep_transform: sizes %d x %d must be multiples of 16, at most 16384, "
+                               "nx*ny <= 2^26", nx, ny);
+    ep_status pst = check_precision(precision, "ep_transform");
+    if (pst != EP_OK) return pst;
+    const size_t nb = (size_t)nx * ny;
+    for (size_t k = 0; k < nb; ++k)
+        if (!std::isfinite(in[k])) return fail(EP_ENAN, "ep_transform: non-finite input at %zu", k);
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) return fail(EP_EDEVICE, "ep_transform: no HIP device");
+    return run_transform(row_kind, col_kind, nx, ny, in, out, precision);
 }

@@ -44,9 +44,13 @@ constexpr int kSortCellsPerThread = 16;
 // one extra evaluation per rejection, so they are skipped.
 constexpr int kModelBackoffMax = 32;
 // Captured evaluations fork the WA gradient onto a second branch only from this
-// many placeable cells on: the fork and join cost ~15 us per trial, more than the
-// overlap saves on small instances.
+// many placeable cells on (ep_params.fork_min_cells = 0): the fork and join cost
+// ~15 us per trial, more than the overlap saves on small instances.
 constexpr int64_t kForkMinCells = 400000;
+inline int64_t fork_min_cells(const ep_params &p)
+{
+    return p.fork_min_cells > 0 ? (int64_t)p.fork_min_cells : kForkMinCells;
+}
 // Step-model records kept per context between ep_take_step_records calls.
 constexpr size_t kMaxStepRecords = (size_t)1 << 21;
 // Default HPWL reference of the lambda schedule: this many bin widths per net.
@@ -231,6 +235,9 @@ ep_status field_init(Ctx &c);   // builds transform matrices
 // FP64 device solve of the current map; max-norm relative error of ex/ey against it.
 ep_status field_error(Ctx &c, double *err_ex, double *err_ey);
 void field_free(Ctx &c);
+// One reference-convention 1-D/2-D DCT / IDCT / IDXST of a host nx x ny map on
+// the product kernels (ep_transform; arguments validated by the caller).
+ep_status run_transform(int row_kind, int col_kind, int nx, int ny, const float *in, float *out, int precision);
 
 struct Ctx {
     int device = 0;

@@ -1133,4 +1133,147 @@ int batch_field_k_step(const BatchField *bf, int precision)
     return precision == EP_PREC_FP32 ? kSkBatchF32[v] : kSkBatchSplit[v];
 }
 
+// ---------------------------------------------------------------- standalone transforms
+//
+// One reference-convention transform on the product path (ep_transform): the
+// transformed axis is staged contiguous, IDCT/IDXST rows get x_0/2, IDXST rows
+// are reversed (IDXST(x)_k = (-1)^k IDCT(x_0, x_{n-1}, ..., x_1)_k, DCT.cpp:64-82),
+// and one product with the forward or inverse cosine matrix transforms every row.
+
+namespace {
+
+constexpr int kTrBlock = 256;
+
+// X[r][k] = rowsign(r) * h(k) * src[r][k'] over R rows of length n, where
+// k' = n - k for reversed rows (k > 0), h(0) = 1/2 when `halve`, and rowsign(r)
+// = (-1)^r when `altsign` (the previous pass was an IDXST).  `tr`: the source is
+// the n x R map whose columns are the rows (src[k'][r]).  Tracks max |X| in maxbits.
+__global__ __launch_bounds__(kTrBlock) void k_tr_prep(const float *__restrict__ src, float *__restrict__ X,
+                                                      int R, int n, int tr, int reverse, int halve, int altsign,
+                                                      uint32_t *__restrict__ maxbits)
+{
+    float m = 0.0f;
+    const int64_t total = (int64_t)R * n;
+    for (int64_t e = (int64_t)blockIdx.x * kTrBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kTrBlock) {
+        const int r = (int)(e / n), k = (int)(e - (int64_t)r * n);
+        const int kk = (reverse && k > 0) ? n - k : k;
+        float v = tr ? src[(int64_t)kk * R + r] : src[(int64_t)r * n + kk];
+        if (halve && k == 0) v *= 0.5f;
+        if (altsign && (r & 1)) v = -v;
+        X[e] = v;
+        m = fmaxf(m, fabsf(v));
+    }
+    track_max(maxbits, m);
+}
+
+// Product output C[f][r] (n x R) -> Y: sign (-1)^f when `altsign`; Y[r][f] when
+// `tr`, else in the same layout.
+__global__ __launch_bounds__(kTrBlock) void k_tr_post(const float *__restrict__ C, float *__restrict__ Y, int R,
+                                                      int n, int altsign, int tr)
+{
+    const int64_t total = (int64_t)R * n;
+    for (int64_t e = (int64_t)blockIdx.x * kTrBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kTrBlock) {
+        const int f = (int)(e / R), r = (int)(e - (int64_t)f * R);
+        float v = C[e];
+        if (altsign && (f & 1)) v = -v;
+        Y[tr ? (int64_t)r * n + f : e] = v;
+    }
+}
+
+__global__ void k_tr_zero(uint32_t *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+int tr_grid(int64_t n)
+{
+    const int64_t g = (n + kTrBlock - 1) / kTrBlock;
+    return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 4096);
+}
+
+}  // namespace
+
+ep_status run_transform(int row_kind, int col_kind, int nx, int ny, const float *in, float *out, int precision)
+{
+    const size_t nb = (size_t)nx * ny;
+    float *d_in = nullptr, *d_x = nullptr, *d_c = nullptr, *d_out = nullptr;
+    uint32_t *d_max = nullptr;
+    HalfMats mats[2];
+    ep_status st = EP_OK;
+    hipStream_t s = nullptr;
+    auto cleanup = [&]() {
+        void *p[] = {d_in, d_x, d_c, d_out, d_max};
+        for (void *q : p)
+            if (q) (void)hipFree(q);
+        free_halves(mats[0]);
+        free_halves(mats[1]);
+    };
+    if (hipMalloc(&d_in, nb * 4) != hipSuccess || hipMalloc(&d_x, nb * 4) != hipSuccess ||
+        hipMalloc(&d_c, nb * 4) != hipSuccess || hipMalloc(&d_out, nb * 4) != hipSuccess ||
+        hipMalloc(&d_max, 4 * sizeof(uint32_t)) != hipSuccess) {
+        cleanup();
+        return fail(EP_ENOMEM, "ep_transform: hipMalloc failed");
+    }
+    if (hipMemcpy(d_in, in, nb * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        return fail(EP_EDEVICE, "ep_transform: upload failed");
+    }
+    hipLaunchKernelGGL(k_tr_zero, dim3(1), dim3(64), 0, s, d_max, 4);
+    // One pass: rows of X (R x n, staged from src) through the product into C (n x R).
+    auto pass = [&](int kind, int n, int R, const float *src, int tr, int altsign, HalfMats &hm,
+                    uint32_t *bmax, uint32_t *cmax) -> ep_status {
+        const int slot = kind == EP_TR_DCT ? MAT_CX : MAT_CXT;  // forward / inverse cosine
+        ep_status e = upload_halves(n, slot, hm);
+        if (e != EP_OK) return e;
+        hipLaunchKernelGGL(k_tr_prep, dim3(tr_grid((int64_t)R * n)), dim3(kTrBlock), 0, s, src, d_x, R, n, tr,
+                           kind == EP_TR_IDXST ? 1 : 0, kind == EP_TR_DCT ? 0 : 1, altsign, bmax);
+        GemmArgs a{};
+        set_operand(a, 0, hm, slot, precision);
+        a.M = n;
+        a.N = R;
+        a.K = n;
+        a.B[0] = d_x;
+        a.C[0] = d_c;
+        a.bmax = bmax;
+        a.cmax = cmax;
+        gemm(a, 1, kind != EP_TR_DCT, 0, precision, s);
+        return EP_OK;
+    };
+    if (row_kind != EP_TR_NONE && col_kind != EP_TR_NONE) {
+        // rows of in (nx rows of length ny) -> C1[v][i]; then the rows of C1
+        // (ny rows of length nx) -> C2[u][v] = out
+        if ((st = pass(row_kind, ny, nx, d_in, 0, 0, mats[0], d_max + 0, d_max + 1)) != EP_OK) {
+            cleanup();
+            return st;
+        }
+        if ((st = pass(col_kind, nx, ny, d_c, 0, row_kind == EP_TR_IDXST, mats[1], d_max + 2, d_max + 3)) !=
+            EP_OK) {
+            cleanup();
+            return st;
+        }
+        hipLaunchKernelGGL(k_tr_post, dim3(tr_grid((int64_t)nb)), dim3(kTrBlock), 0, s, d_c, d_out, ny, nx,
+                           col_kind == EP_TR_IDXST ? 1 : 0, 0);
+    } else if (row_kind != EP_TR_NONE) {
+        if ((st = pass(row_kind, ny, nx, d_in, 0, 0, mats[0], d_max + 0, d_max + 1)) != EP_OK) {
+            cleanup();
+            return st;
+        }
+        hipLaunchKernelGGL(k_tr_post, dim3(tr_grid((int64_t)nb)), dim3(kTrBlock), 0, s, d_c, d_out, nx, ny,
+                           row_kind == EP_TR_IDXST ? 1 : 0, 1);
+    } else {
+        // columns of in: staged transposed (ny rows of length nx) -> C[u][j] = out
+        if ((st = pass(col_kind, nx, ny, d_in, 1, 0, mats[0], d_max + 0, d_max + 1)) != EP_OK) {
+            cleanup();
+            return st;
+        }
+        hipLaunchKernelGGL(k_tr_post, dim3(tr_grid((int64_t)nb)), dim3(kTrBlock), 0, s, d_c, d_out, ny, nx,
+                           col_kind == EP_TR_IDXST ? 1 : 0, 0);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, nb * 4, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(EP_EDEVICE, "ep_transform: %s", hipGetErrorString(e));
+    return EP_OK;
+}
+
 }  // namespace ep

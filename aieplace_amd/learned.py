@@ -39,7 +39,8 @@ def collect_records(instances, precision=api.EP_PREC_BF16X3, log=print):
     cfg = CONFIGS["stdcell_2m"]
     for k, (n, seed) in enumerate(instances):
         nl = generate("stdcell", seed=seed, n_movable=n)
-        b = bins_for_cells(n) if n < 60_000 else (512 if n < 700_000 else 1024)
+        # the grid that the workloads pair with this size: 0.3-1.5 cells per bin
+        b = bins_for_cells(n) if n < 700_000 else 1024
         p = placer_params(dict(cfg, bins=b), precision=precision, use_graph=1)
         with Placer(nl, p) as pl:
             res = pl.run()

@@ -84,6 +84,9 @@ typedef struct ep_params {
     int32_t use_graph;        /* replay one gradient evaluation as a captured hipGraph */
     int32_t learned_step;     /* 1: step length proposed by the loaded MLP, verified by
                                  the backtracking condition (fallback to the search) */
+    int32_t fork_min_cells;   /* captured evaluations run the WA gradient on a second
+                                 branch from this many placeable cells on (results are
+                                 identical either way); 0 = the default, 400000     */
 } ep_params;
 
 /* One Nesterov iteration's outcome (replaces printIterationResults'
@@ -206,6 +209,22 @@ ep_status ep_set_step_model(ep_ctx *ctx, const float *weights, int32_t n_weights
  * net_size*8 partials per group in the same lane order (compute_partials.cpp:11-63). */
 ep_status ep_wa_packet(const float *packet_in, int32_t n_in, float gamma, float *partials_out,
                        int32_t n_out);
+
+/* ---- one transform on the MFMA product path (the AIE DCT / IDCT / IDXST graphs'
+ *      DCT_in -> DCT_out, vck5000/aie/src/density/DCTGraph.h:41-42, IDCTGraph.h:40-41,
+ *      IDXSTGraph.h:42-43; the host's DCT_naive / IDCT_naive / IDXST_naive and
+ *      transpose, vck5000/host/src/DCT.cpp:5-82) ----------------------------- */
+enum { EP_TR_NONE = -1, EP_TR_DCT = 0, EP_TR_IDCT = 1, EP_TR_IDXST = 2 };
+/* in, out: nx x ny row-major host arrays (out may equal in).  row_kind is applied
+ * to every row (length ny), then col_kind to every column (length nx); EP_TR_NONE
+ * leaves that axis as it is (not both).  Unscaled reference conventions:
+ *   DCT    y_k = sum_n x_n cos(pi k (n + 1/2) / N)
+ *   IDCT   y_k = x_0 / 2 + sum_{n>=1} x_n cos(pi n (k + 1/2) / N)
+ *   IDXST  y_k = (-1)^k IDCT(x_0, x_{N-1}, ..., x_1)_k
+ * nx and ny are multiples of 16 (at most 16384, nx * ny <= 2^26); precision is
+ * EP_PREC_*.  Allocates its device buffers per call. */
+ep_status ep_transform(int32_t row_kind, int32_t col_kind, int32_t nx, int32_t ny, const float *in,
+                       float *out, int32_t precision);
 
 #ifdef __cplusplus
 }

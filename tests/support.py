@@ -37,6 +37,20 @@ def oracle_params(params: api.EpParams, bw: float, n_nets: int) -> dict:
                 max_iters=params.max_iters, init_disp_bins=params.init_disp_bins)
 
 
+def hpwl_at_overflow(overflow, hpwl, target=0.10, window=12):
+    """HPWL of a run at overflow exactly `target`: log HPWL fitted linearly in
+    the overflow over the run's last `window` iterations (the run stops at the
+    first iteration with overflow <= target, so they straddle it).  Near the
+    stop HPWL grows ~0.2% per iteration while the overflow falls by ~0.0005, so
+    the final HPWL depends on which iteration crosses the target first; the
+    value at matched overflow does not."""
+    overflow = np.asarray(overflow, np.float64)
+    hpwl = np.asarray(hpwl, np.float64)
+    a = max(0, len(hpwl) - window)
+    c = np.polyfit(overflow[a:], np.log(hpwl[a:]), 1)
+    return float(np.exp(np.polyval(c, target)))
+
+
 def netlist_lefdef(nl, units=1000, name="gen"):
     """LEF (one macro per cell) and DEF text for a netlist; cells [movable | fixed]."""
     lef, comps, nets = [], [], []

@@ -161,3 +161,61 @@ def test_batch_reset_restarts_from_initial_placement():
         r2 = second["instance_iterations"]
         assert abs(r1 - r2) <= max(6, 0.05 * r1), (r1, r2)
 
+
+
+def test_forked_batch_matches_single():
+    """The batch graph with its WA gradient on a second captured branch (the
+    form the 64-instance workload takes while >= 400k cells run), forced here
+    at a few thousand cells: every instance still reproduces its
+    single-instance host-controlled run exactly."""
+    counts, bins = [6_000, 11_000, 8_000], [128, 128, 128]
+    say(f"case forked batch counts={counts}")
+    nls, p = small_batch(counts, bins, fork_min_cells=1)
+    with BatchPlacer(nls, bins, p) as bp:
+        info = bp.run()
+        res = bp.results()
+        xb = [bp.positions(i) for i in range(len(counts))]
+    assert info["finished"] == len(counts)
+    for i, (nl, b) in enumerate(zip(nls, bins)):
+        with A.Placer(nl, single_params(p, b)) as pl:
+            r = pl.run()
+            xs, ys = pl.get_positions()
+        say(f"  inst {i}: batch it={res['iterations'][i]} ev={res['evals'][i]} | single it={r['iterations']} "
+            f"ev={r['evals']}")
+        assert res["iterations"][i] == r["iterations"] and res["evals"][i] == r["evals"], i
+        assert np.array_equal(xs, xb[i][0]) and np.array_equal(ys, xb[i][1]), i
+
+
+@pytest.mark.parametrize("precision", [api.EP_PREC_FP32, api.EP_PREC_BF16X3])
+def test_batch_64_workload(precision):
+    """The batched workload itself: 64 instances of 5k-150k cells (log-uniform,
+    seeds 1-64) on 128/256/512 bins, one graph replay per four trials with
+    compaction 64 -> 32 -> ... -> 1.  Every instance reaches overflow <= 0.10,
+    and a sample (smallest, largest and two mid-size instances) reproduces its
+    single-instance host-controlled run exactly."""
+    from aieplace_amd.batch import batch_workload
+    nls, bins = batch_workload(64)
+    counts = [nl.n_mov for nl in nls]
+    say(f"case batch 64 precision={precision}: {sum(counts)} movable cells, bins "
+        f"{ {b: bins.count(b) for b in sorted(set(bins))} }")
+    p = A.placer_params(A.CONFIGS["stdcell_200k"], precision=precision)
+    with BatchPlacer(nls, bins, p) as bp:
+        info = bp.run()
+        res = bp.results()
+        order = np.argsort(counts)
+        sample = [int(order[0]), int(order[-1]), int(order[len(order) // 3]), int(order[2 * len(order) // 3])]
+        xb = {i: bp.positions(i) for i in sample}
+    say(f"  {info['instance_iterations']} instance-iterations, {info['launches']} replays, "
+        f"{info['seconds']:.3f} s ({info['instance_iterations'] / info['seconds']:.0f} instance-iterations/s), "
+        f"iterations {res['iterations'].min()}..{res['iterations'].max()}")
+    assert info["finished"] == 64
+    assert (res["overflow"] <= p.stop_overflow + 1e-9).all(), np.where(res["overflow"] > p.stop_overflow)
+    assert (res["iterations"] < p.max_iters).all()
+    for i in sample:
+        with A.Placer(nls[i], single_params(p, bins[i])) as pl:
+            r = pl.run()
+            xs, ys = pl.get_positions()
+        say(f"  inst {i} ({counts[i]} cells, {bins[i]}^2): batch it={res['iterations'][i]} ev={res['evals'][i]} "
+            f"| single it={r['iterations']} ev={r['evals']}")
+        assert res["iterations"][i] == r["iterations"] and res["evals"][i] == r["evals"], i
+        assert np.array_equal(xs, xb[i][0]) and np.array_equal(ys, xb[i][1]), i

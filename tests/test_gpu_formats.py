@@ -116,3 +116,95 @@ def test_placement_fp16x3():
     say(f"case placement fp16x3: fp32 hpwl {a['hpwl']:.1f}, fp16x3 hpwl {b['hpwl']:.1f}")
     assert b["overflow"] <= 0.10
     assert b["hpwl"] == pytest.approx(a["hpwl"], rel=5e-3)
+
+
+# ------------------------------------------------------------------ transforms (ep_transform)
+
+DENS = os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures", "density")
+
+
+def rd_density(name):
+    with open(os.path.join(DENS, name)) as f:
+        return np.array([float(l) for l in f if l.strip()])
+
+
+def transform(x, row_kind, col_kind, precision=api.EP_PREC_FP32):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    api.check(api.lib.ep_transform(row_kind, col_kind, x.shape[0], x.shape[1], api.fptr(x), api.fptr(out),
+                                   precision))
+    return out
+
+
+KIND = {"dct": api.EP_TR_DCT, "idct": api.EP_TR_IDCT, "idxst": api.EP_TR_IDXST}
+
+
+def test_transform_golden_replay():
+    """The AIE density graphs' fixtures (vck5000/aie/golden_data/density, fed
+    through DCT_in -> DCT_out, DCTGraph.h:41-42 / IDCTGraph.h:40-41 /
+    IDXSTGraph.h:42-43) through the MFMA product path, at the fixtures' own
+    tolerance (values printed with two decimals; input.dat holds rows 0-7)."""
+    say("case transform golden replay (16 x 16)")
+    inp = rd_density("input.dat")[0::2][:128].reshape(8, 16)
+    dct = rd_density("dct_output.dat").reshape(16, 16, 2)[:, :, 0]
+    d2 = rd_density("dct_2d_output.dat").reshape(16, 16, 2)[:, :, 0]
+    idct = rd_density("idct_output.dat").reshape(16, 16, 2)[:, :, 0]
+    idxst = rd_density("idxst_output.dat").reshape(16, 16, 2)[:, :, 0]
+    idct2 = rd_density("idct_2d_output.dat").reshape(16, 16, 2)[:, :, 0]
+    idxst2 = rd_density("idxst_2d_output.dat").reshape(16, 16, 2)[:, :, 0]
+    for prec in (api.EP_PREC_FP32, api.EP_PREC_BF16X3, api.EP_PREC_FP16X3):
+        x = np.zeros((16, 16), np.float32)
+        x[:8] = inp
+        got = transform(x, api.EP_TR_DCT, api.EP_TR_NONE, prec)
+        err = np.abs(got[:8] - dct[:8]).max()
+        say(f"  precision {prec}: dct rows max err {err:.4f}")
+        assert err <= 0.01
+        for kind, ref in (("idct", idct), ("idxst", idxst)):
+            got = transform(d2, KIND[kind], api.EP_TR_NONE, prec)
+            err = np.abs(got - ref).max()
+            say(f"  precision {prec}: {kind} rows max err {err:.4f}")
+            assert err <= 0.1
+        for kind, ref in (("idct", idct2), ("idxst", idxst2)):
+            got = transform(d2, KIND[kind], KIND[kind], prec)
+            err = np.abs(got - ref).max()
+            say(f"  precision {prec}: {kind} 2-D max err {err:.4f}")
+            assert err <= 0.2
+        # IDCT2(DCT2(input)) = (N/2)^2 input on the known rows
+        back = transform(transform(x, api.EP_TR_DCT, api.EP_TR_DCT, prec), api.EP_TR_IDCT, api.EP_TR_IDCT, prec)
+        np.testing.assert_allclose(back[:8] / 64.0, inp, atol=1e-4)
+
+
+@pytest.mark.parametrize("shape", [(16, 32), (128, 256), (512, 512), (1024, 96)])
+def test_transform_matches_oracle(oracle, shape):
+    """Every kind pair, one axis or both, against the FP64 oracle transforms
+    (eo_transform2d): FP32 within 1e-5 and the split forms within 5e-5 of the
+    result's max-norm."""
+    say(f"case transform vs oracle {shape}")
+    rng = np.random.default_rng(shape[0] + shape[1])
+    x = rng.standard_normal(shape).astype(np.float32)
+    names = ("dct", "idct", "idxst")
+    for rk in names:
+        for ck in (None,) + names:
+            if ck is None:
+                ref = np.stack([oracle.transform1d(rk, r) for r in x.astype(np.float64)])
+            else:
+                ref = oracle.transform2d(x.astype(np.float64), ck, rk)
+            for prec, tol in ((api.EP_PREC_FP32, 1e-5), (api.EP_PREC_BF16X3, 5e-5), (api.EP_PREC_FP16X3, 5e-5)):
+                got = transform(x, KIND[rk], api.EP_TR_NONE if ck is None else KIND[ck], prec)
+                err = np.abs(got - ref).max() / np.abs(ref).max()
+                assert err <= tol, (rk, ck, prec, err)
+    # one axis along the columns only
+    ref = np.stack([oracle.transform1d("idxst", c) for c in x.astype(np.float64).T]).T
+    got = transform(x, api.EP_TR_NONE, api.EP_TR_IDXST)
+    assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-5
+
+
+def test_transform_rejects_bad_arguments():
+    x = np.zeros((16, 16), np.float32)
+    out = np.zeros_like(x)
+    for args in ((api.EP_TR_NONE, api.EP_TR_NONE, 16, 16), (5, api.EP_TR_DCT, 16, 16),
+                 (api.EP_TR_DCT, api.EP_TR_NONE, 24, 16), (api.EP_TR_DCT, api.EP_TR_NONE, 0, 16)):
+        assert api.lib.ep_transform(*args, api.fptr(x), api.fptr(out), api.EP_PREC_FP32) == api.EP_EINVAL
+    x[3, 3] = np.nan
+    assert api.lib.ep_transform(api.EP_TR_DCT, api.EP_TR_NONE, 16, 16, api.fptr(x), api.fptr(out),
+                                api.EP_PREC_FP32) == api.EP_ENAN

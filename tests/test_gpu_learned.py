@@ -139,6 +139,63 @@ def test_learned_step_ablation_committed_model():
         assert lrn["hpwl"] <= max(spread) * 1.005
 
 
+def test_learned_runs_start_fresh():
+    """The model's rejection backoff and the proposal counters belong to one run:
+    a second run from the same start (after set_positions) repeats the first one
+    exactly.  A model that is always rejected ends its first run inside a
+    backoff, so any state carried over would change when the second run's
+    proposals happen (and with them the evaluation counts)."""
+    say("case learned runs start fresh")
+    w = np.zeros(api.MLP_NWEIGHTS, np.float32)
+    w[16:32] = 1.0
+    w[32], w[33] = 6.0, 1.0  # always proposes 10^6 bin widths: always rejected
+    for model in (w, learned.load()):
+        nl, p = _instance(52, use_graph=1, learned_step=1, precision=api.EP_PREC_BF16X3)
+        with A.Placer(nl, p) as pl:
+            pl.set_step_model(model)
+            x0, y0 = pl.get_positions()
+            r1 = pl.run(log_cap=4000)
+            pl.set_positions(x0, y0)
+            r2 = pl.run(log_cap=4000)
+        say(f"  run 1: it {r1['iterations']} ev {r1['evals']} proposals {r1['learned_proposals']} "
+            f"accepted {r1['learned_accepted']}; run 2: it {r2['iterations']} ev {r2['evals']} "
+            f"proposals {r2['learned_proposals']} accepted {r2['learned_accepted']}")
+        for r in (r1, r2):
+            assert 0 < r["learned_proposals"] <= r["iterations"]
+        for k in ("iterations", "evals", "learned_proposals", "learned_accepted", "hpwl"):
+            assert r1[k] == r2[k], k
+        assert r1["log"] == r2["log"]
+
+
+def test_learned_1m_workload():
+    """The learned-step workload: 1M cells on 1024x1024 bins, split-BF16
+    transforms, with and without the committed model (device-controlled runs).
+    Reports acceptance, gradient evaluations per iteration and HPWL against the
+    same form without the model."""
+    import os
+    if not os.path.exists(learned.DEFAULT_WEIGHTS):
+        pytest.skip("no committed step model")
+    say("case learned_1m: 1M cells, 1024^2 bins, split-BF16")
+    nl = A.generate("stdcell", seed=1, n_movable=1_000_000)
+    p = A.placer_params(A.CONFIGS["learned_1m"], precision=api.EP_PREC_BF16X3, use_graph=1)
+    with A.Placer(nl, p) as pl:
+        plain = pl.run()
+    p.learned_step = 1
+    with A.Placer(nl, p) as pl:
+        pl.set_step_model(learned.load())
+        lrn = pl.run()
+    rate = lrn["learned_accepted"] / max(1, lrn["learned_proposals"])
+    say(f"  without model: {plain['iterations']} it, {plain['evals']} evals "
+        f"({plain['evals'] / plain['iterations']:.3f}/it), hpwl {plain['hpwl']:.5e}, {plain['seconds']:.3f} s")
+    say(f"  with model:    {lrn['iterations']} it, {lrn['evals']} evals "
+        f"({lrn['evals'] / lrn['iterations']:.3f}/it), hpwl {lrn['hpwl']:.5e}, {lrn['seconds']:.3f} s; "
+        f"{lrn['learned_proposals']} proposals, acceptance {rate:.3f}")
+    assert plain["overflow"] <= 0.10 and lrn["overflow"] <= 0.10
+    assert 0 < lrn["learned_proposals"] <= lrn["iterations"]
+    assert rate > 0.5
+    assert lrn["hpwl"] <= plain["hpwl"] * 1.005
+
+
 def test_corrupted_model_falls_back():
     """A model proposing absurd steps is rejected by the check every time and
     the plain search still converges."""
