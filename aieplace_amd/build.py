@@ -16,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libeplace.so")
 RUNNER = os.path.join(HERE, "eplace_run")
+EXAMPLE = os.path.join(HERE, "placer_example")
 
 SOURCES = ["k_place.hip", "k_field.hip", "k_mlp.hip", "k_batch.hip", "ep_api.cpp", "batch.cpp",
            "generator.cpp", "lefdef.cpp", "legalize.cpp"]
@@ -68,6 +69,15 @@ def build(verbose: bool = False) -> str:
         if r.returncode != 0:
             raise RuntimeError(f"eplace_run build failed:\n{r.stdout}\n{r.stderr}")
         os.replace(RUNNER + ".tmp", RUNNER)
+    # the C++ Placer example (include/eplace_placer.hpp over the C ABI)
+    ex_src = os.path.join(CSRC, "placer_example.cpp")
+    if os.path.exists(ex_src):
+        cmd = [hipcc(), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"), ex_src,
+               "-o", EXAMPLE + ".tmp", "-L", HERE, "-leplace", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"placer_example build failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(EXAMPLE + ".tmp", EXAMPLE)
     if verbose:
         print(f"built {LIB}")
     return LIB

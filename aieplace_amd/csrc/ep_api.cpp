@@ -1354,10 +1354,12 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
         double bytes, flops;
     };
     const double M = c->Mx;
-    // symmetric transform products: 3 Mx My (Mx + My) flops per solve (6 M^3
-    // square, half the dense 12 M^3); the split forms run three products each
+    // algorithmic flops of one solve: six dense transform products (the 2-D DCT,
+    // then two inverse 2-D transforms), 6 Mx My (Mx + My) = 12 M^3 on a square
+    // grid.  The symmetric half-products execute half of that in FP32; the split
+    // forms execute three products each (1.5x, at the 16-bit rate).
     const double Mxd = c->Mx, Myd = c->My;
-    const double gemm_flops = 3.0 * Mxd * Myd * (Mxd + Myd) * (c->prm.precision == EP_PREC_FP32 ? 1.0 : 3.0);
+    const double gemm_flops = 6.0 * Mxd * Myd * (Mxd + Myd);
     // reads 6 maps, writes 7; the halves of the six matrices: 1.5 (Mx^2 + My^2) floats
     const double field_bytes = 4.0 * 13 * nb + 4.0 * 1.5 * (Mxd * Mxd + Myd * Myd);
     Item items[] = {

@@ -18,6 +18,20 @@ from . import api
 from .netlist import Netlist
 
 
+def hpwl_at_overflow(overflow, hpwl, target: float = 0.10, window: int = 12) -> float:
+    """HPWL of a run at overflow exactly `target`, from its per-iteration log:
+    log HPWL fitted linearly in the overflow over the last `window` iterations
+    (the run stops at the first iteration with overflow <= target, so they
+    straddle it).  Near the stop HPWL grows ~0.2% per iteration while the
+    overflow falls by ~0.0005, so the final HPWL depends on which iteration
+    crosses the target first; the value at matched overflow does not."""
+    overflow = np.asarray(overflow, np.float64)
+    hpwl = np.asarray(hpwl, np.float64)
+    a = max(0, len(hpwl) - window)
+    c = np.polyfit(overflow[a:], np.log(hpwl[a:]), 1)
+    return float(np.exp(np.polyval(c, target)))
+
+
 class Placer:
     def __init__(self, netlist: Netlist, params: Optional[api.EpParams] = None, **kw):
         self.netlist = netlist

@@ -365,11 +365,13 @@ def mfma(args) -> str:
         lines += ["", f"Transform products together (the four dispatches of one field solve): MFMA busy "
                       f"{100 * busy_m / (4 * busy_c):.1f}% of SIMD cycles; {q['avg_us']:.1f} us per solve "
                       f"(ep_profile, HIP events, the driver run again without the profiler) for "
-                      f"{q['flops']:.4g} flops = {tf:.1f} TFLOP/s, "
+                      f"{q['flops']:.4g} algorithmic flops (the six dense transform products, "
+                      f"6 Mx My (Mx + My)) = {tf:.1f} TFLOP/s, "
                       f"{100 * tf / PEAK_TFLOPS[args.precision]:.1f}% of the dense "
                       f"{'FP32' if args.precision == 0 else '16-bit'} MFMA peak "
-                      f"({PEAK_TFLOPS[args.precision]:g} TFLOP/s; the split forms count each of the three "
-                      "products' flops).", ""]
+                      f"({PEAK_TFLOPS[args.precision]:g} TFLOP/s).  The symmetric half-products execute "
+                      "half of these flops in FP32; the split forms execute three products each, 1.5x "
+                      "of them at the 16-bit rate.", ""]
         lines.append(f"mfma field_transforms busy {busy_m / (4 * busy_c):.4f} tflops {tf:.2f}")
     path = os.path.join(PROFILES, f"mfma_{args.workload}_{prec}.md")
     with open(path, "w") as f:

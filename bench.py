@@ -31,6 +31,7 @@ METRIC = "sec to overflow<=0.10 @2M cells, mixed+learned vs plain FP32; HPWL gap
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_BF16_MFMA_TFLOPS = 2500.0
+FILLER_SEEDS = (7, 8, 9, 10)
 
 
 def log(*a):
@@ -67,6 +68,24 @@ def run_form(A, nl, cfg, form, steps, warmup):
         log(f"[{form['name']}] step {k}: {r['iterations']} it, {r['evals']} evals, "
             f"{r['seconds']:.3f}s (wall {r['wall']:.3f}s), hpwl {r['hpwl']:.6e}, tau {r['overflow']:.4f}")
     return pl, runs
+
+
+def seed_spread(A, nl, cfg, pl, seeds=FILLER_SEEDS):
+    """Untimed: the same form from the initial placement with the fillers of
+    each seed (the final HPWL of one run moves by ~1% with them); HPWL at
+    overflow exactly 0.10 of each run (hpwl_at_overflow)."""
+    from aieplace_amd.netlist import plan_fillers
+    from aieplace_amd.placer import hpwl_at_overflow
+    n_mov = nl.n_mov
+    out = []
+    for fs in seeds:
+        _, _, _, fx, fy = plan_fillers(nl, A.placer_params(cfg, filler_seed=fs))
+        pl.set_positions(np.concatenate([nl.x[:n_mov], fx]).astype(np.float32),
+                         np.concatenate([nl.y[:n_mov], fy]).astype(np.float32))
+        r = pl.run(log_cap=5000)
+        at = hpwl_at_overflow([l["overflow"] for l in r["log"]], [l["hpwl"] for l in r["log"]])
+        out.append(dict(seed=fs, iterations=r["iterations"], hpwl=r["hpwl"], hpwl_at_010=at))
+    return out
 
 
 def precision_report(A, nl, cfg, every=50):
@@ -106,8 +125,10 @@ def profile(pl, reps=20):
 
 def roofline_of(stats, traffic=None, precision_bf16=False):
     """Dominant item of ep_profile (longest call) against its roofline.
-    `traffic`: {item: measured HBM bytes per call} from the committed
-    rocprofv3 --pmc summary of the same workload, or None."""
+    Transform products count their algorithmic flops: the six dense products
+    of a solve, 6 Mx My (Mx + My) (12 M^3 square); the split forms are held to
+    the 16-bit peak.  `traffic`: {item: measured HBM bytes per call} from the
+    committed rocprofv3 --pmc summary of the same workload, or None."""
     dom = max(stats, key=lambda s: s["avg_us"])
     sec = dom["avg_us"] * 1e-6
     tr = (traffic or {}).get(dom["name"])
@@ -122,12 +143,40 @@ def roofline_of(stats, traffic=None, precision_bf16=False):
                      frac=round(ach / PEAK_HBM_GBS, 4), traffic=tr)
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: the cores this job is given
+    (OMP_NUM_THREADS, 16 per GPU on the measurement hosts), else every CPU."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
 def cpu_baseline(A, nl, cfg, gpu_evals, budget_s):
-    """FP64 oracle on the host cores over a bounded number of iterations of the
-    same workload; per-evaluation time projected to the GPU plain form's count."""
+    """CPU timings of the same workload on this host, each a bounded sample
+    projected to the GPU plain form's gradient-evaluation count:
+      * the FP64 oracle (oracle/eplace_oracle.c, OpenMP): a few Nesterov
+        iterations, and one field solve (separable products with tables);
+      * the field by the reference's own RePlAce FFT solver (oracle/_ref, built
+        from python/AIEmath/fft_replace when the reference tree was present):
+        O(M^2 log M), one thread;
+      * the field as the reference's host path evaluates it (DCT_naive /
+        IDCT_naive / IDXST_naive, one cos() per term): a sample of rows of one
+        pass, scaled to the six passes of M rows of a solve.
+    `value` is the fastest CPU evaluation: the oracle's evaluation with the
+    faster of its own field solve and the FFT solve."""
     from oracle import oracle as O
     from aieplace_amd.netlist import plan_fillers
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     O.set_threads(threads)
     p = A.placer_params(cfg)
     n_fill, fw, fh, fx, fy = plan_fillers(nl, p)
@@ -151,20 +200,45 @@ def cpu_baseline(A, nl, cfg, gpu_evals, budget_s):
     _, _, info = d.place(x0, y0, op)
     t1 = time.perf_counter() - t0
     per_eval = t1 / max(1, info["evals"])
-    iters = max(1, int(budget_s / per_eval) - 3)
-    iters = min(iters, 20)
+    iters = min(max(1, int(budget_s / per_eval) - 3), 20)
     if t1 < budget_s / 2 and iters > 1:
         op["max_iters"] = iters
         t0 = time.perf_counter()
         _, _, info = d.place(x0, y0, op)
         t1 = time.perf_counter() - t0
         per_eval = t1 / max(1, info["evals"])
-    proj = per_eval * gpu_evals
-    return dict(value=round(proj, 2), unit="s", cores=threads, kind="port",
-                sample=(f"FP64 oracle (oracle/eplace_oracle.c, OpenMP) on the same {nl.n_mov}-cell "
-                        f"instance: {op['max_iters']} Nesterov iterations = {info['evals']} gradient "
-                        f"evaluations in {t1:.1f}s ({per_eval:.3f}s/eval), projected to the GPU plain "
-                        f"form's {gpu_evals} evaluations to overflow<=0.10"))
+    # the field alone, three ways, on the initial map
+    M = p.bins_x
+    rho = (d.density(x0, y0, "stretched", 0, d.n_place) + p.target_density * d.density_fixed()) / (bw * bw)
+    def best_of(fn, reps=3):
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            t.append(time.perf_counter() - t0)
+        return min(t)
+
+    t_field = best_of(lambda: O.field(rho))
+    t_fft = best_of(lambda: O.ref_fft_field(rho.astype(np.float32))) if O.ref_available() else None
+    sample = np.ascontiguousarray(rho[:max(threads, 64)], np.float32)
+    rows = sample.shape[0]
+    t_naive = best_of(lambda: O.rows_naive(sample, "dct"), 1) * 6.0 * M / rows
+    rest = max(per_eval - t_field, 0.0)  # the evaluation without its field solve
+    best = rest + (min(t_field, t_fft) if t_fft is not None else t_field)
+    proj = best * gpu_evals
+    comp = dict(oracle_eval_s=round(per_eval, 4), oracle_field_s=round(t_field, 4),
+                fft_field_s=None if t_fft is None else round(t_fft, 4), naive_cos_field_s=round(t_naive, 2),
+                oracle_projection_s=round(per_eval * gpu_evals, 1),
+                naive_cos_projection_s=round((rest + t_naive) * gpu_evals, 1))
+    return dict(value=round(proj, 2), unit="s", cores=threads, kind="port", cpu_model=cpu_model(),
+                host_cpus=os.cpu_count(), components=comp,
+                sample=(f"same {nl.n_mov}-cell instance on {threads} host threads: FP64 oracle "
+                        f"{op['max_iters']} Nesterov iterations = {info['evals']} gradient evaluations in "
+                        f"{t1:.1f}s ({per_eval:.3f}s/eval, field {t_field:.3f}s of it); field by the "
+                        f"reference's RePlAce FFT solver "
+                        + (f"{t_fft:.3f}s (one thread)" if t_fft is not None else "not built")
+                        + f"; naive-cos field (reference DCT_naive structure) {t_naive:.1f}s from {rows} rows of "
+                        f"one pass.  value = the evaluation with the faster field x the GPU plain form's {gpu_evals} evaluations"))
 
 
 def batched_report(A, n=64, sample=16):
@@ -226,6 +300,7 @@ def main():
     ap.add_argument("--no-precision-report", action="store_true")
     ap.add_argument("--forms", default="plain,mixed,mixed_learned")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-seed-spread", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -268,6 +343,12 @@ def main():
 
     head = next((f for f in ("mixed_learned", "mixed", "plain") if f in results), "plain")
     plain = results["plain"]
+    spread = {}
+    if not args.no_seed_spread:
+        for name in ("plain", head) if head != "plain" else ("plain",):
+            spread[name] = seed_spread(A, nl, cfg, placers[name])
+            log(f"[{name}] hpwl at overflow 0.10 over filler seeds: "
+                + ", ".join(f"{q['seed']}: {q['hpwl_at_010']:.5e}" for q in spread[name]))
     out = dict(metric=METRIC, value=results[head]["seconds"], unit="s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(results[head]["seconds"] * 1000, 2),
                higher_is_better=False, scaling="strong", vs_baseline=None,
@@ -280,17 +361,30 @@ def main():
         m = results[head]
         out["speedup_vs_plain"] = round(plain["seconds"] / m["seconds"], 3)
         out["hpwl_gap_pct"] = round(100.0 * (m["hpwl"] - plain["hpwl"]) / plain["hpwl"], 3)
+        if spread:
+            a = float(np.mean([q["hpwl_at_010"] for q in spread["plain"]]))
+            b = float(np.mean([q["hpwl_at_010"] for q in spread[head]]))
+            out["hpwl_gap_pct_seed_mean"] = round(100.0 * (b - a) / a, 3)
+            out["hpwl_seed_runs"] = {k: [dict(seed=q["seed"], iterations=q["iterations"],
+                                              hpwl_at_010=round(q["hpwl_at_010"], 1)) for q in v]
+                                     for k, v in spread.items()}
 
     if not args.no_precision_report:
         out["field_error_vs_fp64"] = precision_report(A, nl, cfg)
     from aieplace_amd.profiling import read_traffic
-    stats = profile(placers["plain"])
-    dom, roof = roofline_of(stats, read_traffic(args.workload, api.EP_PREC_FP32))
-    out["roofline"] = dict(kernel=dom["name"], **roof)
-    if roof["traffic"] is not None:
-        out["roofline"]["traffic_source"] = f"profiles/traffic_{args.workload}_fp32.md"
-    out["kernels"] = {s["name"]: dict(avg_us=round(s["avg_us"], 2), launches=s["launches"])
-                      for s in stats}
+    kern = {}
+    for name, key in ((head, "roofline"), ("plain", "roofline_plain")) if head != "plain" else (("plain", "roofline"),):
+        bf16 = forms_all[name]["precision"] != api.EP_PREC_FP32
+        stats = profile(placers[name])
+        tag = "bf16x3" if bf16 else "fp32"
+        dom, roof = roofline_of(stats, read_traffic(args.workload, forms_all[name]["precision"]), bf16)
+        out[key] = dict(kernel=dom["name"], form=name, **roof)
+        if roof["traffic"] is not None:
+            out[key]["traffic_source"] = f"profiles/traffic_{args.workload}_{tag}.md"
+        kern[name] = {q["name"]: dict(avg_us=round(q["avg_us"], 2), launches=q["launches"]) for q in stats}
+    out["roofline_flops"] = ("transform products count algorithmic flops: the six dense products of a "
+                             "solve, 6 Mx My (Mx + My) = 12 M^3; split-BF16 held to the BF16 peak")
+    out["kernels"] = kern
     if not args.no_batched:
         out["batched"] = batched_report(A)
     if not args.no_cpu:

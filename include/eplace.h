@@ -190,7 +190,8 @@ typedef struct ep_kernel_stat {
     int32_t launches;   /* launches of this group per gradient evaluation        */
     double avg_us;      /* event time per group launch                           */
     double bytes;       /* algorithmic (unique) HBM bytes per group launch       */
-    double flops;       /* algorithmic flops per group launch (MFMA work)        */
+    double flops;       /* algorithmic flops per group launch: the dense transform
+                           products, 6 Mx My (Mx + My) per field solve          */
 } ep_kernel_stat;
 ep_status ep_profile(ep_ctx *ctx, int32_t reps, ep_kernel_stat *out, int32_t cap, int32_t *n);
 

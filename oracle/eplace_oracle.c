@@ -439,6 +439,39 @@ static double *trig_table(int M, int sine, int transpose)
     return t;
 }
 
+/* DCT.cpp:24-82: DCT_naive accumulates in double, IDCT_naive in float, one
+ * cos() per term; IDXST_naive reverses x_1..x_{N-1}, applies IDCT_naive and
+ * negates the odd outputs. */
+void eo_rows_naive(const float *in, int rows, int n, int kind, float *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < rows; ++r) {
+        const float *x = in + (size_t)r * n;
+        float *y = out + (size_t)r * n;
+        if (kind == 0) {
+            for (int k = 0; k < n; ++k) {
+                double sum = 0;
+                for (int j = 0; j < n; ++j) sum += x[j] * cos(M_PI / n * (j + .5) * k);
+                y[k] = (float)sum;
+            }
+            continue;
+        }
+        float tmp[16384];
+        const float *src = x;
+        if (kind == 2) {
+            tmp[0] = x[0];
+            for (int j = 1; j < n; ++j) tmp[j] = x[n - j];
+            src = tmp;
+        }
+        for (int k = 0; k < n; ++k) {
+            float sum = 0;
+            for (int j = 1; j < n; ++j) sum += src[j] * (float)cos(M_PI / n * (k + .5) * j);
+            y[k] = 0.5f * src[0] + sum;
+            if (kind == 2 && (k & 1)) y[k] = -y[k];
+        }
+    }
+}
+
 /* fft.cpp:130-191 spectral step, shared by both field routines. */
 static void spectral(double *a, int Mx, int My, double wy_scale, double *ph, double *ehx,
                      double *ehy)

@@ -109,6 +109,13 @@ void eo_idst(const double *x, int n, double *y);
  * the second index, col_kind along the first. */
 void eo_transform2d(const double *in, int nx, int ny, int col_kind, int row_kind, double *out);
 
+/* Row transforms with one cos() call per term, as the reference's host path
+ * evaluates them: kind 0 DCT_naive (double sum, DCT.cpp:24-40), 1 IDCT_naive
+ * (float sum, DCT.cpp:45-59), 2 IDXST_naive (reversal + IDCT + sign,
+ * DCT.cpp:64-82).  rows x n, row-major; OpenMP over rows.  Used to time the
+ * reference-faithful CPU field (six such passes of M rows per solve). */
+void eo_rows_naive(const float *in, int rows, int n, int kind, float *out);
+
 /* Poisson field solve, RePlAce convention (python/AIEmath/fft_replace/fft.cpp:118-212):
  *   a = DCT2(rho) * 4/(Mx My), halved on u=0 and v=0;  wx = pi u/Mx,
  *   wy = pi v/My * wy_scale;  phi^ = a/(wx^2+wy^2) (0 at DC);

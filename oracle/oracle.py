@@ -192,6 +192,17 @@ def field(rho, wy_scale: float = 1.0, direct: bool = False):
     return phi, ex, ey
 
 
+def rows_naive(x, kind: str):
+    """Row transforms with one cos() per term (the reference's DCT_naive /
+    IDCT_naive / IDXST_naive); rows of a float32 (rows, n) array, n <= 16384."""
+    L = lib()
+    L.eo_rows_naive.argtypes = [_fp, C.c_int, C.c_int, C.c_int, _fp]
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros_like(x)
+    L.eo_rows_naive(x, x.shape[0], x.shape[1], {"dct": 0, "idct": 1, "idxst": 2}[kind], out)
+    return out
+
+
 def cell_bins_f32(x, y, w, h, xl, yl, bw, bh, mx, my, stretch=True):
     x = np.ascontiguousarray(x, dtype=np.float32)
     y = np.ascontiguousarray(y, dtype=np.float32)

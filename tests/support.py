@@ -5,6 +5,7 @@ import numpy as np
 
 from aieplace_amd import api
 from aieplace_amd.netlist import Netlist, plan_fillers
+from aieplace_amd.placer import hpwl_at_overflow  # noqa: F401  (shared with bench.py)
 
 
 def oracle_design(O, nl: Netlist, params: api.EpParams):
@@ -37,18 +38,14 @@ def oracle_params(params: api.EpParams, bw: float, n_nets: int) -> dict:
                 max_iters=params.max_iters, init_disp_bins=params.init_disp_bins)
 
 
-def hpwl_at_overflow(overflow, hpwl, target=0.10, window=12):
-    """HPWL of a run at overflow exactly `target`: log HPWL fitted linearly in
-    the overflow over the run's last `window` iterations (the run stops at the
-    first iteration with overflow <= target, so they straddle it).  Near the
-    stop HPWL grows ~0.2% per iteration while the overflow falls by ~0.0005, so
-    the final HPWL depends on which iteration crosses the target first; the
-    value at matched overflow does not."""
-    overflow = np.asarray(overflow, np.float64)
-    hpwl = np.asarray(hpwl, np.float64)
-    a = max(0, len(hpwl) - window)
-    c = np.polyfit(overflow[a:], np.log(hpwl[a:]), 1)
-    return float(np.exp(np.polyval(c, target)))
+def nudged_start(x, member: int):
+    """Start `member` of an ensemble: member 0 is x itself; member m > 0 moves a
+    seeded half of the coordinates (rng seed m) to the next FP32 value up."""
+    x = np.array(x, dtype=np.float32)
+    if member > 0:
+        m = np.random.default_rng(member).random(len(x)) < 0.5
+        x[m] = np.nextafter(x[m], np.float32(np.inf))
+    return x
 
 
 def netlist_lefdef(nl, units=1000, name="gen"):

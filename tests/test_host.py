@@ -197,3 +197,18 @@ def test_batch_workload_sizes_log_uniform():
     logs = np.log(counts)
     assert logs.max() - logs.min() > 0.8 * (np.log(150_000) - np.log(5_000))
     assert {bins_for_cells(c) for c in counts} == {128, 256, 512}
+
+
+def test_integration_quotes_the_compiled_cpp_example():
+    """INTEGRATION.md shows the C++ Placer example that the build compiles
+    (aieplace_amd/csrc/placer_example.cpp, between its markers) verbatim, and
+    the compiled example rejects bad arguments before touching a device."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = open(os.path.join(root, "aieplace_amd", "csrc", "placer_example.cpp")).read()
+    body = src.split("// ---- BEGIN INTEGRATION EXAMPLE\n")[1].split("    // ---- END INTEGRATION EXAMPLE")[0]
+    doc = open(os.path.join(root, "INTEGRATION.md")).read()
+    assert body in doc
+    exe = os.path.join(root, "aieplace_amd", "placer_example")
+    assert subprocess.run([exe, "torus"], capture_output=True).returncode == 2
+    assert subprocess.run([exe, "mesh", "--bogus", "1"], capture_output=True).returncode == 2
