@@ -1501,6 +1501,30 @@ extern "C" ep_status ep_wa_packet(const float *in, int32_t n_in, float gamma, fl
     return st;
 }
 
+extern "C" ep_status ep_wa_packet_noalloc(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out)
+{
+    int net_size = 0, n_nets = 0, groups = 0;
+    int64_t n_data = 0;
+    ep_status st = check_packet(in, n_in, gamma, out, n_out, "ep_wa_packet_noalloc", &net_size, &n_nets, &groups,
+                                &n_data);
+    if (st != EP_OK || n_data == 0) return st;
+    if (n_data > EP_WA_PACKET_NOALLOC_MAX)
+        return fail(EP_EINVAL, "ep_wa_packet_noalloc: packet holds %lld data values; the fixed staging holds %d",
+                    (long long)n_data, EP_WA_PACKET_NOALLOC_MAX);
+    float *stage = nullptr;
+    if ((st = wa_packet_stage(&stage)) != EP_OK) return st;
+    float *d_in = stage, *d_out = stage + EP_WA_PACKET_NOALLOC_MAX;
+    // the kernel writes every partial of every group (zeros for absent lanes), so
+    // nothing a previous call left in the staging is read or returned
+    if (hipMemcpy(d_in, in + 4, sizeof(float) * n_data, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(EP_EDEVICE, "ep_wa_packet_noalloc: upload failed");
+    launch_wa_packet(d_in, net_size, n_nets, groups, 1.0f / gamma, d_out, nullptr);
+    if ((st = check_launch()) != EP_OK) return st;
+    if (hipMemcpy(out, d_out, sizeof(float) * n_data, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(EP_EDEVICE, "ep_wa_packet_noalloc: download failed");
+    return EP_OK;
+}
+
 extern "C" ep_status ep_transform(int32_t row_kind, int32_t col_kind, int32_t nx, int32_t ny, const float *in,
                                   float *out, int32_t precision)
 {

@@ -230,6 +230,9 @@ void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipSt
 void launch_mlp_propose(Ctx &c, hipStream_t s);
 void launch_wa_packet(const float *data, int net_size, int n_nets, int groups, float ginv,
                       float *out, hipStream_t s);
+// Device address of the current GPU's fixed packet staging (2 x
+// EP_WA_PACKET_NOALLOC_MAX floats: data, then partials), looked up once per device.
+ep_status wa_packet_stage(float **stage);
 void launch_cell_bins(Ctx &c, const float2 *pos, int stretched, int32_t *out, hipStream_t s);
 ep_status field_init(Ctx &c);   // builds transform matrices
 // FP64 device solve of the current map; max-norm relative error of ex/ey against it.

@@ -12,6 +12,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <mutex>
 
 #include "ep_ctx.h"
 #include "ep_mlp.h"
@@ -879,6 +880,37 @@ void launch_wa_packet(const float *data, int net_size, int n_nets, int groups, f
     const int threads = groups * 8;
     hipLaunchKernelGGL(k_wa_packet, dim3((threads + 255) / 256), dim3(256), 0, s, data, net_size,
                        n_nets, groups, ginv, out);
+}
+
+// Device staging of ep_wa_packet_noalloc: packet data in the first half, the
+// partials in the second.  Each GPU holds its own copy until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress gives on
+// that device (the array's name is a host-side stand-in, not a device address).
+constexpr int64_t kPacketStage = EP_WA_PACKET_NOALLOC_MAX;
+static __device__ __align__(256) float scratch[2 * kPacketStage];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_stage_mu;
+float *g_stage[kMaxDevices] = {};
+}  // namespace
+
+ep_status wa_packet_stage(float **stage)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return fail(EP_EDEVICE, "ep_wa_packet_noalloc: no current HIP device (or index >= %d)", kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    if (!g_stage[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(EP_EDEVICE, "ep_wa_packet_noalloc: hipGetSymbolAddress on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_stage[dev] = static_cast<float *>(p);
+    }
+    *stage = g_stage[dev];
+    return EP_OK;
 }
 
 void launch_cell_bins(Ctx &c, const float2 *pos, int stretched, int32_t *out, hipStream_t s)

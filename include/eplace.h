@@ -210,6 +210,16 @@ ep_status ep_set_step_model(ep_ctx *ctx, const float *weights, int32_t n_weights
  * net_size*8 partials per group in the same lane order (compute_partials.cpp:11-63). */
 ep_status ep_wa_packet(const float *packet_in, int32_t n_in, float gamma, float *partials_out,
                        int32_t n_out);
+/* ep_wa_packet without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (tight packet loops, code beside
+ * graph capture, harnesses that forbid hipMalloc).  The packet is staged in one
+ * fixed array per GPU (2 x EP_WA_PACKET_NOALLOC_MAX floats = 8 MiB, held until
+ * the process ends); packets with more data values are rejected with EP_EINVAL.
+ * All calls on one GPU share that array, so calls that overlap on one GPU race.
+ * Results equal ep_wa_packet's bit for bit. */
+#define EP_WA_PACKET_NOALLOC_MAX (1 << 20)
+ep_status ep_wa_packet_noalloc(const float *packet_in, int32_t n_in, float gamma, float *partials_out,
+                               int32_t n_out);
 
 /* ---- one transform on the MFMA product path (the AIE DCT / IDCT / IDXST graphs'
  *      DCT_in -> DCT_out, vck5000/aie/src/density/DCTGraph.h:41-42, IDCTGraph.h:40-41,
