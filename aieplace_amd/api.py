@@ -43,7 +43,7 @@ class EpParams(C.Structure):
         ("max_backtrack", C.c_int32), ("stop_overflow", C.c_float), ("max_iters", C.c_int32),
         ("init_disp_bins", C.c_float), ("use_fillers", C.c_int32), ("filler_seed", C.c_uint64),
         ("precision", C.c_int32), ("use_graph", C.c_int32), ("learned_step", C.c_int32),
-        ("fork_min_cells", C.c_int32),
+        ("fork_min_cells", C.c_int32), ("cell_order", C.c_int32),
     ]
 
 

@@ -67,6 +67,7 @@ struct ep_batch {
     int n = 0;
     std::vector<BInst> inst;
     std::vector<int32_t> n_fix;
+    std::vector<std::vector<int32_t>> cell_order;  // per instance: stored movable cell -> caller's index
     std::vector<float2> init_pos;   // all cells, concatenated
     int64_t total_cells = 0;
 
@@ -359,6 +360,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     std::vector<float2> pin_off;
     std::vector<float> fixmap;
     std::vector<BBucket> bks;
+    std::vector<float> means;  // per instance: constant mean area per bin of the field map
     std::vector<int32_t> pre_cell{0}, pre_sort{0}, pre_bin{0}, pre_wa{0}, pre_dens[3];
     for (int v = 0; v < 3; ++v) pre_dens[v].push_back(0);
     int64_t cell_off = 0, map_off = 0, pin_base = 0, net_base = 0, cpin_base = 0, wa_base = 0;
@@ -394,6 +396,8 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
         b->dens_wcap = std::max(b->dens_wcap, dp.wcap);
         b->inst.push_back(I);
         b->n_fix.push_back(L.n_fix);
+        b->cell_order.push_back(L.cell_order);
+        means.push_back(field_map_mean(L, p->target_density));
         const int64_t span = (L.n_all + kCellAlign - 1) / kCellAlign * kCellAlign;
         if (cell_off + span > INT32_MAX || map_off + (int64_t)bins[i] * bins[i] > INT32_MAX / 2 ||
             pin_base + L.n_pins > INT32_MAX) {
@@ -526,6 +530,7 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
         return fail(EP_ENOMEM, "ep_batch_create: pinned host allocation failed");
     }
     EPB_TRY(upload(b->d_inst, b->inst));
+    EPB_TRY(upload(b->map_mean, means));
     EPB_TRY(upload(b->wh, wh));
     EPB_TRY(upload(b->net_start, net_start));
     EPB_TRY(upload(b->pin_cell, pin_cell));
@@ -741,8 +746,9 @@ static ep_status pull_pairs(ep_batch *b, int32_t i, bool grad, float *x, float *
     std::vector<float2> h(I.n_place);
     EPB_HIP(hipMemcpy(h.data(), src, sizeof(float2) * I.n_place, hipMemcpyDeviceToHost));
     for (int k = 0; k < I.n_place; ++k) {
-        x[k] = h[k].x;
-        y[k] = h[k].y;
+        const int32_t o = caller_cell(b->cell_order[i], k);
+        x[o] = h[k].x;
+        y[o] = h[k].y;
     }
     return EP_OK;
 }

@@ -203,6 +203,7 @@ extern "C" void ep_default_params(ep_params *p)
     p->use_graph = 0;
     p->learned_step = 0;
     p->fork_min_cells = 0;
+    p->cell_order = 1;
 }
 
 static ep_status check_precision(int32_t precision, const char *who)
@@ -231,6 +232,7 @@ ep_status ep::validate_netlist(const ep_netlist *nl, const ep_params *p)
     if (!(p->gamma_bins > 0.f)) return fail(EP_EINVAL, "ep_create: gamma_bins must be > 0");
     if (p->max_backtrack < 1) return fail(EP_EINVAL, "ep_create: max_backtrack must be >= 1");
     if (p->fork_min_cells < 0) return fail(EP_EINVAL, "ep_create: fork_min_cells must be >= 0");
+    if (p->cell_order != 0 && p->cell_order != 1) return fail(EP_EINVAL, "ep_create: cell_order must be 0 or 1");
     ep_status pst = check_precision(p->precision, "ep_create");
     if (pst != EP_OK) return pst;
     const int n_all = nl->n_mov + nl->n_fix;
@@ -368,6 +370,66 @@ double ep::gamma_at(const ep_params &p, double bw, double tau)
     return g;
 }
 
+// Movable cells in breadth-first order over the cell-net incidence (nets of
+// degree >= 2), each connected component from a pseudo-peripheral cell (the
+// last cell a first sweep reaches), components in the order of their first
+// cell.  Cells of one net then get nearby indices whatever the caller's order,
+// which keeps the WA kernel's pin -> cell gathers and the per-cell pin sums
+// cache-local (the same role as a bandwidth-reducing matrix ordering).
+static void connectivity_order(const ep_netlist *nl, std::vector<int32_t> &order)
+{
+    const int n = nl->n_mov, N = nl->n_nets;
+    std::vector<int32_t> cs(n + 1, 0);
+    for (int e = 0; e < N; ++e) {
+        if (nl->net_start[e + 1] - nl->net_start[e] < 2) continue;
+        for (int q = nl->net_start[e]; q < nl->net_start[e + 1]; ++q)
+            if (nl->pin_cell[q] < n) cs[nl->pin_cell[q] + 1]++;
+    }
+    for (int k = 0; k < n; ++k) cs[k + 1] += cs[k];
+    std::vector<int32_t> cn(std::max(1, cs[n])), fill(cs.begin(), cs.end() - 1);
+    for (int e = 0; e < N; ++e) {
+        if (nl->net_start[e + 1] - nl->net_start[e] < 2) continue;
+        for (int q = nl->net_start[e]; q < nl->net_start[e + 1]; ++q)
+            if (nl->pin_cell[q] < n) cn[fill[nl->pin_cell[q]]++] = e;
+    }
+    std::vector<char> seen_c(n, 0), seen_n(N, 0);
+    std::vector<int32_t> nets_hit;
+    order.clear();
+    order.reserve(n);
+    auto sweep = [&](int s) {
+        size_t head = order.size();
+        order.push_back(s);
+        seen_c[s] = 1;
+        while (head < order.size()) {
+            const int c = order[head++];
+            for (int k = cs[c]; k < cs[c + 1]; ++k) {
+                const int e = cn[k];
+                if (seen_n[e]) continue;
+                seen_n[e] = 1;
+                nets_hit.push_back(e);
+                for (int q = nl->net_start[e]; q < nl->net_start[e + 1]; ++q) {
+                    const int c2 = nl->pin_cell[q];
+                    if (c2 < n && !seen_c[c2]) {
+                        seen_c[c2] = 1;
+                        order.push_back(c2);
+                    }
+                }
+            }
+        }
+    };
+    for (int s = 0; s < n; ++s) {
+        if (seen_c[s]) continue;
+        const size_t mark = order.size();
+        nets_hit.clear();
+        sweep(s);
+        const int far = order.back();
+        for (size_t k = mark; k < order.size(); ++k) seen_c[order[k]] = 0;
+        for (int e : nets_hit) seen_n[e] = 0;
+        order.resize(mark);
+        sweep(far);
+    }
+}
+
 ep_status ep::build_layout(const ep_netlist *nl, const ep_params *p, HostLayout &L)
 {
     ep_status st = validate_netlist(nl, p);
@@ -392,12 +454,22 @@ ep_status ep::build_layout(const ep_netlist *nl, const ep_params *p, HostLayout 
     L.n_place = L.n_mov + L.n_fill;
     L.n_all = L.n_place + L.n_fix;
 
+    // ---- movable cell order: stored cell k is the caller's cell_order[k]
+    std::vector<int32_t> inv;  // caller's movable index -> stored index
+    if (p->cell_order == 1 && L.n_mov > 1) {
+        connectivity_order(nl, L.cell_order);
+        inv.assign(L.n_mov, 0);
+        for (int k = 0; k < L.n_mov; ++k) inv[L.cell_order[k]] = k;
+    }
+    auto stored = [&](int32_t cell) { return cell < L.n_mov && !inv.empty() ? inv[cell] : cell; };
+
     // ---- host cell arrays [mov | fill | fix]
     L.wh.resize(L.n_all);
     L.pos.resize(L.n_all);
     for (int k = 0; k < L.n_mov; ++k) {
-        L.wh[k] = make_float2(nl->w[k], nl->h[k]);
-        L.pos[k] = make_float2(nl->x[k], nl->y[k]);
+        const int src = caller_cell(L.cell_order, k);
+        L.wh[k] = make_float2(nl->w[src], nl->h[src]);
+        L.pos[k] = make_float2(nl->x[src], nl->y[src]);
     }
     {
         std::vector<float> fxs(std::max(1, L.n_fill)), fys(std::max(1, L.n_fill));
@@ -449,7 +521,7 @@ ep_status ep::build_layout(const ep_netlist *nl, const ep_params *p, HostLayout 
         std::vector<int32_t> key(N, 0);
         for (int n = 0; n < N; ++n) {
             int32_t m = INT32_MAX;
-            for (int q = nl->net_start[n]; q < nl->net_start[n + 1]; ++q) m = std::min(m, nl->pin_cell[q]);
+            for (int q = nl->net_start[n]; q < nl->net_start[n + 1]; ++q) m = std::min(m, stored(nl->pin_cell[q]));
             key[n] = m;
         }
         for (int b = 0; b < kMaxBuckets; ++b)
@@ -477,7 +549,7 @@ ep_status ep::build_layout(const ep_netlist *nl, const ep_params *p, HostLayout 
             L.net_start[net_w] = pin_w;
             L.net_perm.push_back(n);
             for (int q = nl->net_start[n]; q < nl->net_start[n + 1]; ++q) {
-                int32_t cell = nl->pin_cell[q];
+                int32_t cell = stored(nl->pin_cell[q]);
                 if (cell >= L.n_mov) cell += L.n_fill;  // fixed cells follow the fillers
                 L.pin_cell[pin_w] = cell;
                 L.pin_off[pin_w] = make_float2(nl->pin_dx[q], nl->pin_dy[q]);
@@ -540,6 +612,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     c->n_pins = L.n_pins;
     c->n_pins_mov = L.n_pins_mov;
     c->net_perm = L.net_perm;
+    c->cell_order = L.cell_order;
     c->buckets = L.buckets;
     c->wa_blocks = L.wa_blocks;
     c->wa_part_blocks = std::min(L.wa_blocks, kWaGridCap);
@@ -636,6 +709,10 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
         return fail(EP_EDEVICE, "ep_create: hipMemset failed");
     }
     EP_TRY(field_init(*c));
+    {
+        const float mean = field_map_mean(L, p->target_density);
+        EP_TRY(upload(c->map_mean, &mean, 1));
+    }
 #undef EP_TRY
     if (hipDeviceSynchronize() != hipSuccess) {
         destroy_ctx(c);
@@ -673,8 +750,9 @@ extern "C" ep_status ep_get_cells(ep_ctx *c, float *w, float *h, float *fix_x, f
     if (c->n_fix)
         EP_HIP(hipMemcpy(pos.data(), c->V[c->cur] + c->n_place, sizeof(float2) * c->n_fix, hipMemcpyDeviceToHost));
     for (int k = 0; k < c->n_all; ++k) {
-        if (w) w[k] = wh[k].x;
-        if (h) h[k] = wh[k].y;
+        const int32_t o = caller_cell(c->cell_order, k);
+        if (w) w[o] = wh[k].x;
+        if (h) h[o] = wh[k].y;
     }
     for (int k = 0; k < c->n_fix; ++k) {
         if (fix_x) fix_x[k] = pos[k].x;
@@ -688,7 +766,10 @@ extern "C" ep_status ep_get_nets(ep_ctx *c, int32_t *net_start, int32_t *pin_cel
 {
     if (!c) return fail(EP_EINVAL, "ep_get_nets: null context");
     if (net_start) EP_HIP(hipMemcpy(net_start, c->net_start, sizeof(int32_t) * (c->n_nets + 1), hipMemcpyDeviceToHost));
-    if (pin_cell && c->n_pins) EP_HIP(hipMemcpy(pin_cell, c->pin_cell, sizeof(int32_t) * c->n_pins, hipMemcpyDeviceToHost));
+    if (pin_cell && c->n_pins) {
+        EP_HIP(hipMemcpy(pin_cell, c->pin_cell, sizeof(int32_t) * c->n_pins, hipMemcpyDeviceToHost));
+        for (int q = 0; q < c->n_pins; ++q) pin_cell[q] = caller_cell(c->cell_order, pin_cell[q]);
+    }
     if ((pin_dx || pin_dy) && c->n_pins) {
         std::vector<float2> o(c->n_pins);
         EP_HIP(hipMemcpy(o.data(), c->pin_off, sizeof(float2) * c->n_pins, hipMemcpyDeviceToHost));
@@ -707,10 +788,11 @@ extern "C" ep_status ep_set_positions(ep_ctx *c, const float *x, const float *y)
     std::vector<float2> p(std::max(1, c->n_place)), wh(std::max(1, c->n_place));
     EP_HIP(hipMemcpy(wh.data(), c->wh, sizeof(float2) * c->n_place, hipMemcpyDeviceToHost));
     for (int k = 0; k < c->n_place; ++k) {
-        if (!std::isfinite(x[k]) || !std::isfinite(y[k]))
-            return fail(EP_ENAN, "ep_set_positions: non-finite position at cell %d", k);
-        p[k] = make_float2(std::min(std::max(x[k], c->xl), c->xh - wh[k].x),
-                           std::min(std::max(y[k], c->yl), c->yh - wh[k].y));
+        const int32_t o = caller_cell(c->cell_order, k);
+        if (!std::isfinite(x[o]) || !std::isfinite(y[o]))
+            return fail(EP_ENAN, "ep_set_positions: non-finite position at cell %d", o);
+        p[k] = make_float2(std::min(std::max(x[o], c->xl), c->xh - wh[k].x),
+                           std::min(std::max(y[o], c->yl), c->yh - wh[k].y));
     }
     EP_HIP(hipMemcpy(c->V[c->cur], p.data(), sizeof(float2) * c->n_place, hipMemcpyHostToDevice));
     EP_HIP(hipMemcpy(c->U[c->cur], p.data(), sizeof(float2) * c->n_place, hipMemcpyHostToDevice));
@@ -724,8 +806,9 @@ extern "C" ep_status ep_get_positions(ep_ctx *c, float *x, float *y)
     std::vector<float2> p(std::max(1, c->n_place));
     EP_HIP(hipMemcpy(p.data(), c->V[c->cur], sizeof(float2) * c->n_place, hipMemcpyDeviceToHost));
     for (int k = 0; k < c->n_place; ++k) {
-        x[k] = p[k].x;
-        y[k] = p[k].y;
+        const int32_t o = caller_cell(c->cell_order, k);
+        x[o] = p[k].x;
+        y[o] = p[k].y;
     }
     return EP_OK;
 }
@@ -796,14 +879,16 @@ extern "C" ep_status ep_get_field(ep_ctx *c, float *ex, float *ey)
     return EP_OK;
 }
 
-static ep_status copy_pairs(const float2 *d, int n, float *ox, float *oy)
+// Per-cell pairs of cells [mov | fill] back to the caller, in the caller's order.
+static ep_status copy_pairs(const ep_ctx *c, const float2 *d, int n, float *ox, float *oy)
 {
     if (!ox && !oy) return EP_OK;
     std::vector<float2> h(std::max(1, n));
     EP_HIP(hipMemcpy(h.data(), d, sizeof(float2) * n, hipMemcpyDeviceToHost));
     for (int k = 0; k < n; ++k) {
-        if (ox) ox[k] = h[k].x;
-        if (oy) oy[k] = h[k].y;
+        const int32_t o = caller_cell(c->cell_order, k);
+        if (ox) ox[o] = h[k].x;
+        if (oy) oy[o] = h[k].y;
     }
     return EP_OK;
 }
@@ -837,7 +922,7 @@ extern "C" ep_status ep_get_wa_grad(ep_ctx *c, float *gx, float *gy)
     launch_cell_grad(*c, c->V[c->cur], nullptr, nullptr, scratch, gw, nullptr, c->stream);
     st = check_launch();
     if (st == EP_OK) st = sync(*c);
-    if (st == EP_OK) st = copy_pairs(gw, c->n_place, gx, gy);
+    if (st == EP_OK) st = copy_pairs(c, gw, c->n_place, gx, gy);
     (void)hipFree(gw);
     return st;
 }
@@ -861,9 +946,9 @@ extern "C" ep_status ep_gradient(ep_ctx *c, float gamma, float lambda, float *gx
     enqueue_eval(*c, c->V[c->cur], nullptr, nullptr, c->G[c->cur], gw, f, c->stream);
     st = check_launch();
     if (st == EP_OK) st = sync(*c);
-    if (st == EP_OK) st = copy_pairs(c->G[c->cur], c->n_place, gx, gy);
-    if (st == EP_OK) st = copy_pairs(gw, c->n_place, gwx, gwy);
-    if (st == EP_OK) st = copy_pairs(f, c->n_place, fx, fy);
+    if (st == EP_OK) st = copy_pairs(c, c->G[c->cur], c->n_place, gx, gy);
+    if (st == EP_OK) st = copy_pairs(c, gw, c->n_place, gwx, gwy);
+    if (st == EP_OK) st = copy_pairs(c, f, c->n_place, fx, fy);
     (void)hipFree(gw);
     (void)hipFree(f);
     if (st != EP_OK) return st;
@@ -883,8 +968,11 @@ extern "C" ep_status ep_cell_bins(ep_ctx *c, int32_t stretched, int32_t *out)
     st = check_launch();
     if (st == EP_OK) st = sync(*c);
     if (st == EP_OK) {
-        hipError_t e = hipMemcpy(out, d, sizeof(int32_t) * 4 * (size_t)c->n_place, hipMemcpyDeviceToHost);
+        std::vector<int32_t> h(4 * (size_t)std::max(1, c->n_place));
+        hipError_t e = hipMemcpy(h.data(), d, sizeof(int32_t) * 4 * (size_t)c->n_place, hipMemcpyDeviceToHost);
         if (e != hipSuccess) st = fail(EP_EDEVICE, "ep_cell_bins: copy failed");
+        for (int k = 0; st == EP_OK && k < c->n_place; ++k)
+            std::memcpy(out + 4 * (size_t)caller_cell(c->cell_order, k), &h[4 * (size_t)k], 4 * sizeof(int32_t));
     }
     (void)hipFree(d);
     return st;
@@ -1408,6 +1496,31 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
             out[k].bytes = it.bytes;
             out[k].flops = it.flops;
         }
+        ++k;
+    }
+    // the density scatter at the other tile sizes (cells per thread), each after
+    // a sort whose block table matches it: the plan's choice is one of them
+    const int cpt0 = c->dens.cpt;
+    for (int cpt : {2, 4, 8, 16}) {
+        if (cpt == cpt0) continue;
+        c->dens.cpt = cpt;
+        launch_sort(*c, c->V[a], false, c->stream);
+        EP_HIP(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; ++r) launch_density(*c, c->stream);
+        EP_HIP(hipEventRecord(e1, c->stream));
+        EP_HIP(hipEventSynchronize(e1));
+        ep_status st = check_launch();
+        if (st == EP_OK && k < cap && out) {
+            float ms = 0;
+            EP_HIP(hipEventElapsedTime(&ms, e0, e1));
+            std::memset(&out[k], 0, sizeof(ep_kernel_stat));
+            std::snprintf(out[k].name, sizeof(out[k].name), "density_scatter_cpt%d", cpt);
+            out[k].launches = 1;
+            out[k].avg_us = 1000.0 * ms / reps;
+            out[k].bytes = 16.0 * np + 16.0 * nb;
+        }
+        c->dens.cpt = cpt0;
+        if (st != EP_OK) return st;
         ++k;
     }
     (void)hipEventDestroy(e0);

@@ -190,6 +190,9 @@ void launch_sort(Ctx &c, Pair<const float2> pos, bool counted, hipStream_t s);
 // Scatters the cells packed by the last launch_sort (same positions).
 void launch_density(Ctx &c, hipStream_t s);
 void launch_overflow(Ctx &c, hipStream_t s);
+// Split-FP16 products: max |field map - mean| of the last overflow pass into
+// field_max[0] (the operand scale of the first product).
+void launch_map_range(Ctx &c, hipStream_t s);
 void launch_wa(Ctx &c, Pair<const float2> pos, hipStream_t s);
 void launch_field(Ctx &c, int precision, hipStream_t s);
 // gw_cell (nullable): per-cell pin-partial sums made by launch_pin_sum; without
@@ -268,6 +271,7 @@ struct Ctx {
     BucketTable buckets{};
     int32_t wa_blocks = 0;
     std::vector<int32_t> net_perm;  // stored net k -> original net index
+    std::vector<int32_t> cell_order;  // stored movable cell k -> caller's index (empty: same order)
 
     // optimiser state
     float2 *V[2] = {nullptr, nullptr}, *U[2] = {nullptr, nullptr}, *G[2] = {nullptr, nullptr};
@@ -284,7 +288,8 @@ struct Ctx {
     float *tA = nullptr, *tB = nullptr;         // transform scratch, 2*Mx*My each
     void *split_mats = nullptr;                 // host handle: FP32 + split transform matrices
     uint32_t *field_max = nullptr;              // [4] running max |operand| for split FP16
-    float *map_mean = nullptr;                  // mean of the field map (removed before the solve)
+    float *map_mean = nullptr;                  // the field map's mean area per bin, a constant of the
+                                                // instance (subtracted before the solve)
     int32_t field_precision_built = -1;
 
     // reductions

@@ -30,7 +30,27 @@ struct HostLayout {
     std::vector<float> fixmap;               // [Mx * My] fixed-cell overlap area
     double hpwl_ref = 0;                     // mu-schedule reference (resolved default)
     double max_sw = 0, max_sh = 0;           // largest smoothed placeable cell (max(w, sqrt2 bw), ...)
+    std::vector<int32_t> cell_order;         // stored movable cell k -> caller's index (empty: identity)
 };
+
+// Caller's index of stored cell k (movable cells may be renumbered; fillers and
+// fixed cells keep their places).
+inline int32_t caller_cell(const std::vector<int32_t> &order, int32_t k)
+{
+    return k < (int32_t)order.size() ? order[k] : k;
+}
+
+// Mean area per bin of the field map: every placeable cell's smoothed rectangle
+// carries its own area, fixed cells rho_t times their overlap.  Cells whose
+// smoothed rectangle crosses the die edge lose the part outside, so the live
+// mean is at most this; the field solve subtracts it before the first product.
+inline float field_map_mean(const HostLayout &L, float rho_t)
+{
+    double a = 0.0, f = 0.0;
+    for (int k = 0; k < L.n_place; ++k) a += (double)L.wh[k].x * L.wh[k].y;
+    for (float v : L.fixmap) f += v;
+    return (float)((a + rho_t * f) / ((double)L.Mx * L.My));
+}
 
 // Validates the netlist and parameters (same checks and messages as ep_create).
 ep_status validate_netlist(const ep_netlist *nl, const ep_params *p);

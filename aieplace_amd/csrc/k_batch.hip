@@ -262,24 +262,22 @@ __global__ __launch_bounds__(kBlock) void kb_overflow(BDev d)
     block_reduce<4, 2>(acc, d.part_ovf + 4 * (size_t)blockIdx.x);
 }
 
-// One block per instance: map mean and max |map - mean| for the field solve.
+// One block per instance: max |map - mean| for the split-FP16 products (the
+// mean is the instance's constant mean area per bin, set at creation).
 __global__ __launch_bounds__(kBlock) void kb_map_stats(BDev d)
 {
     const int i = blockIdx.x;
     if (d.st[i].done) return;
-    const BInst &I = d.inst[i];
-    double v[3] = {0.0, -1e300, -1e300};
+    double v[2] = {-1e300, -1e300};
     for (int b = d.pre_bin[i] + threadIdx.x; b < d.pre_bin[i + 1]; b += kBlock) {
-        v[0] += d.part_ovf[4 * (size_t)b + 1];
-        v[1] = fmax(v[1], d.part_ovf[4 * (size_t)b + 2]);
-        v[2] = fmax(v[2], d.part_ovf[4 * (size_t)b + 3]);
+        v[0] = fmax(v[0], d.part_ovf[4 * (size_t)b + 2]);
+        v[1] = fmax(v[1], d.part_ovf[4 * (size_t)b + 3]);
     }
-    __shared__ double out[3];
-    block_reduce<3, 2>(v, out);
+    __shared__ double out[2];
+    block_reduce<2, 2>(v, out);
     if (threadIdx.x == 0) {
-        const double mean = out[0] / ((double)I.M * I.M);
-        d.map_mean[i] = (float)mean;
-        const double r = fmax(out[1] - mean, -out[2] + mean);
+        const double mean = d.map_mean[i];
+        const double r = fmax(out[0] - mean, -out[1] + mean);
         d.map_max[i] = __float_as_uint((float)fmax(r, 0.0));
     }
 }

@@ -790,12 +790,17 @@ void launch_field(Ctx &c, int precision, hipStream_t s)
     a.wys = (float)(M_PI / c.My * ((double)c.bw / c.bh));
     a.skip = &c.d_scal->halt;  // a finished device-controlled run skips the solve
     auto setA = [&](int z, int m) { set_operand(a, z, fm->half[m], m, precision); };
-    if (f16) hipLaunchKernelGGL(k_clear_max, dim3(1), dim3(64), 0, s, fm->fmax + 1, 3);
+    if (f16) {
+        launch_map_range(c, s);
+        hipLaunchKernelGGL(k_clear_max, dim3(1), dim3(64), 0, s, fm->fmax + 1, 3);
+    }
     // G1: T1t[v][i] = Cy * rho^T
     a.M = c.My; a.N = c.Mx; a.K = c.My;
     setA(0, MAT_CY); a.B[0] = c.map_field; a.C[0] = c.tA;
     a.bmax = fm->fmax + 0; a.cmax = f16 ? fm->fmax + 1 : nullptr;
-    a.bbias = c.map_mean;  // the DC mode carries no field; removing it keeps FP32 sums small
+    // the DC mode carries no field, so subtracting any constant changes only the
+    // dropped DC coefficient; subtracting the mean keeps the FP32 and 16-bit sums small
+    a.bbias = c.map_mean;
     gemm(a, 1, false, 0, precision, s);
     a.bbias = nullptr;
     // G2: A[u][v] = Cx * T1t^T, spectral -> Ehx (tB), Ehy (tB + nb)

@@ -182,26 +182,25 @@ __global__ __launch_bounds__(kBlock) void k_overflow(unsigned long long *__restr
     block_reduce<4, 2>(acc, part + 4 * blockIdx.x);
 }
 
-// Map mean and centred range for the field solve: mean -> mean_out (float),
-// max |map - mean| -> field_max0 (float bits).  One block.
-__global__ __launch_bounds__(kBlock) void k_map_stats(const double *__restrict__ part, int blocks,
-                                                      int nb, float *__restrict__ mean_out,
+// Centred range of the field map for the split-FP16 products: max |map - mean|
+// -> field_max0 (float bits), from k_overflow's per-block max and -min and the
+// context's constant map mean.  One block.
+__global__ __launch_bounds__(kBlock) void k_map_range(const double *__restrict__ part, int blocks,
+                                                      const float *__restrict__ mean_in,
                                                       uint32_t *__restrict__ field_max0,
                                                       const DevScalars *__restrict__ sc)
 {
     if (sc->halt) return;
-    double v[3] = {0.0, -1e300, -1e300};
+    double v[2] = {-1e300, -1e300};
     for (int b = threadIdx.x; b < blocks; b += kBlock) {
-        v[0] += part[4 * b + 1];
-        v[1] = fmax(v[1], part[4 * b + 2]);
-        v[2] = fmax(v[2], part[4 * b + 3]);
+        v[0] = fmax(v[0], part[4 * b + 2]);
+        v[1] = fmax(v[1], part[4 * b + 3]);
     }
-    __shared__ double out[3];
-    block_reduce<3, 2>(v, out);
+    __shared__ double out[2];
+    block_reduce<2, 2>(v, out);
     if (threadIdx.x == 0) {
-        const double mean = out[0] / nb;
-        *mean_out = (float)mean;
-        const double r = fmax(out[1] - mean, -out[2] + mean);
+        const double mean = *mean_in;
+        const double r = fmax(out[0] - mean, -out[1] + mean);
         *field_max0 = __float_as_uint((float)fmax(r, 0.0));
     }
 }
@@ -795,6 +794,7 @@ void launch_density(Ctx &c, hipStream_t s)
     };
     if (dp.cpt == 2) go(k_density<2>);
     else if (dp.cpt == 8) go(k_density<8>);
+    else if (dp.cpt == 16) go(k_density<16>);
     else go(k_density<4>);
 }
 
@@ -806,8 +806,12 @@ void launch_overflow(Ctx &c, hipStream_t s)
     hipLaunchKernelGGL(k_overflow, dim3(c.ovf_blocks), dim3(kBlock), 0, s, c.acc_field, c.acc_mov,
                        c.map_fix, unit, c.prm.target_density, cap, nb, c.map_field, c.map_mov,
                        c.part + c.part_ovf_off, c.d_scal);
-    hipLaunchKernelGGL(k_map_stats, dim3(1), dim3(kBlock), 0, s, c.part + c.part_ovf_off,
-                       c.ovf_blocks, nb, c.map_mean, c.field_max, c.d_scal);
+}
+
+void launch_map_range(Ctx &c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_map_range, dim3(1), dim3(kBlock), 0, s, c.part + c.part_ovf_off, c.ovf_blocks,
+                       c.map_mean, c.field_max, c.d_scal);
 }
 
 void launch_wa(Ctx &c, Pair<const float2> pos, hipStream_t s)

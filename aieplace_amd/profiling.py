@@ -11,6 +11,10 @@
       matrix-core utilisation of the transform products (one --pmc pass of SQ
       counters): SQ_VALU_MFMA_BUSY_CYCLES / (4 SIMDs x SQ_BUSY_CU_CYCLES) per
       kernel, with the products' achieved TFLOP/s -> profiles/mfma_<workload>_<prec>.md
+  python -m aieplace_amd.profiling timeline [--workload stdcell_2m] [--precision 0]
+      kernel trace of one placement split at each Nesterov trial: trial wall
+      time, kernels' busy time (so the idle gaps between them) and each
+      kernel's start offset and duration -> profiles/timeline_<workload>_<prec>.md
   python -m aieplace_amd.profiling batch   [--instances 64] [--precision 0]
       kernel-trace statistics of the batched engine on the 64-instance
       workload (two full runs: warm-up and measured) -> profiles/kernels_batch<n>_<prec>.md
@@ -80,7 +84,7 @@ def drive(args) -> None:
     p = A.placer_params(cfg, precision=args.precision, use_graph=1)
     with A.Placer(nl, p) as pl:
         r = pl.run()
-        prof = pl.profile(args.reps)
+        prof = pl.profile(args.reps) if args.reps > 0 else []
     print(json.dumps({"run": {k: r[k] for k in ("iterations", "evals", "seconds", "hpwl", "overflow")},
                       "profile": prof}), flush=True)
 
@@ -167,6 +171,69 @@ def stats(args) -> str:
         for q in res["profile"]:
             lines.append(f"| {q['name']} | {q['launches']} | {q['avg_us']:.2f} | {q['bytes']:.4g} | {q['flops']:.4g} |")
     path = os.path.join(PROFILES, f"kernels_{args.workload}_{prec}.md")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
+def timeline(args) -> str:
+    """Kernel trace of one placement (no ep_profile repetitions), split at the
+    Nesterov trial kernel: per trial the wall time from one k_trial start to
+    the next, the union of the kernels' busy intervals in it (both captured
+    branches), so the idle time between kernels, and each kernel's mean
+    duration and start offset within the trial."""
+    out_dir = os.path.join(args.scratch, f"timeline_{args.workload}_{args.precision}")
+    cmd = [sys.executable, "-u", "-m", "aieplace_amd.profiling", "drive", "--workload", args.workload,
+           "--precision", str(args.precision), "--reps", "0"]
+    if args.from_trace:  # summarise a trace this command wrote before (e.g. on another host)
+        full = ["rocprofv3", "--kernel-trace", "--output-format", "csv", "-d", out_dir, "-o", "run", "--"] + cmd
+    else:
+        full, _ = _rocprof(["--kernel-trace"], cmd, out_dir, args.timeout)
+    rows = _csv(out_dir, "kernel_trace.csv")
+    ev = sorted(((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _short(r["Kernel_Name"])) for r in rows))
+    starts = [k for k, e in enumerate(ev) if e[2].startswith("k_trial")]
+    trials = []
+    for a, b in zip(starts[:-1], starts[1:]):
+        win = ev[a:b]
+        t0, t1 = win[0][0], ev[b][0]
+        busy, cur_s, cur_e = 0, None, None
+        for s0, e0, _ in win:
+            s0, e0 = max(s0, t0), min(e0, t1)
+            if cur_e is None or s0 > cur_e:
+                if cur_e is not None:
+                    busy += cur_e - cur_s
+                cur_s, cur_e = s0, e0
+            else:
+                cur_e = max(cur_e, e0)
+        if cur_e is not None:
+            busy += cur_e - cur_s
+        trials.append((t1 - t0, busy, [(n, s0 - t0, e0 - s0) for s0, e0, n in win]))
+    # the converged part of the run: the last half of the trials, without the
+    # trials of the final replay that return at once after the stop
+    med = sorted(t[0] for t in trials)[len(trials) // 2]
+    tail = [t for t in trials[len(trials) // 2:] if t[0] > 0.3 * med]
+    wall = sum(t[0] for t in tail) / len(tail) / 1e3
+    busy = sum(t[1] for t in tail) / len(tail) / 1e3
+    per = {}
+    for _, _, ks in tail:
+        seen = {}
+        for n, off, dur in ks:
+            k = n + ("" if n not in seen else f" #{seen[n] + 1}")
+            seen[n] = seen.get(n, 0) + 1
+            per.setdefault(k, []).append((off, dur))
+    prec = PREC_NAME[args.precision]
+    lines = [f"# Kernel timeline: {args.workload}, {prec} transforms", "",
+             "Command (from the repository root):", "", "```", " ".join(full).replace(sys.executable, "python3"),
+             "```", "",
+             f"One device-controlled placement ({len(trials)} trials), split at each Nesterov trial kernel; "
+             f"the figures are means over the last {len(tail)} trials.", "",
+             f"Trial wall time {wall:.1f} us; kernels busy (union of both captured branches) {busy:.1f} us; "
+             f"idle between kernels {wall - busy:.1f} us ({100 * (wall - busy) / wall:.0f}%).", "",
+             "| kernel | start offset us | duration us |", "|---|---:|---:|"]
+    for k, v in sorted(per.items(), key=lambda kv: sum(o for o, _ in kv[1]) / len(kv[1])):
+        lines.append(f"| {k} | {sum(o for o, _ in v) / len(v) / 1e3:.1f} | {sum(d for _, d in v) / len(v) / 1e3:.1f} |")
+    lines.append(f"\ntimeline wall_us {wall:.2f} busy_us {busy:.2f}")
+    path = os.path.join(PROFILES, f"timeline_{args.workload}_{prec}.md")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     return path
@@ -393,7 +460,7 @@ def read_traffic(workload: str, precision: int):
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m aieplace_amd.profiling")
-    ap.add_argument("cmd", choices=["drive", "drive-batch", "stats", "traffic", "mfma", "batch"])
+    ap.add_argument("cmd", choices=["drive", "drive-batch", "stats", "traffic", "mfma", "batch", "timeline"])
     ap.add_argument("--workload", default="stdcell_2m")
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--reps", type=int, default=5)
@@ -402,6 +469,8 @@ def main(argv=None) -> int:
     ap.add_argument("--calib-mib", type=int, default=256, help="traffic: size of the calibration copy")
     ap.add_argument("--scratch", default=os.path.join(ROOT, "gpurun_out", "rocprof"))
     ap.add_argument("--timeout", type=int, default=300)
+    ap.add_argument("--from-trace", action="store_true",
+                    help="timeline: summarise the trace already under --scratch instead of tracing")
     args = ap.parse_args(argv)
     if args.cmd == "drive":
         drive(args)
@@ -411,7 +480,8 @@ def main(argv=None) -> int:
         return 0
     os.makedirs(PROFILES, exist_ok=True)
     t0 = time.time()
-    path = {"stats": stats, "traffic": traffic, "mfma": mfma, "batch": batch_stats}[args.cmd](args)
+    path = {"stats": stats, "traffic": traffic, "mfma": mfma, "batch": batch_stats,
+            "timeline": timeline}[args.cmd](args)
     print(f"wrote {path} in {time.time() - t0:.1f}s", flush=True)
     return 0
 

@@ -129,7 +129,7 @@ def roofline_of(stats, traffic=None, precision_bf16=False):
     of a solve, 6 Mx My (Mx + My) (12 M^3 square); the split forms are held to
     the 16-bit peak.  `traffic`: {item: measured HBM bytes per call} from the
     committed rocprofv3 --pmc summary of the same workload, or None."""
-    dom = max(stats, key=lambda s: s["avg_us"])
+    dom = max((s for s in stats if "_cpt" not in s["name"]), key=lambda s: s["avg_us"])
     sec = dom["avg_us"] * 1e-6
     tr = (traffic or {}).get(dom["name"])
     tr = None if tr is None else round(tr)

@@ -87,6 +87,11 @@ typedef struct ep_params {
     int32_t fork_min_cells;   /* captured evaluations run the WA gradient on a second
                                  branch from this many placeable cells on (results are
                                  identical either way); 0 = the default, 400000     */
+    int32_t cell_order;       /* 1 (default): movable cells stored in breadth-first order
+                                 over the netlist, so that cells of one net sit at nearby
+                                 indices (gather locality for netlists in any order);
+                                 0: stored in the caller's order.  Per-cell arrays passed
+                                 in and out keep the caller's order either way.      */
 } ep_params;
 
 /* One Nesterov iteration's outcome (replaces printIterationResults'

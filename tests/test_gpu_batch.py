@@ -163,6 +163,29 @@ def test_batch_reset_restarts_from_initial_placement():
 
 
 
+def test_renumbered_batch_matches_single():
+    """Instances stored in breadth-first order (cell_order = 1) from netlists in
+    random order: the batch reproduces the single-instance runs, and positions
+    come back in the caller's order."""
+    counts, bins = [6_000, 9_000], [128, 128]
+    say(f"case renumbered batch counts={counts}")
+    nls = [A.generate("stdcell", seed=k + 5, n_movable=c, shuffle_storage=1) for k, c in enumerate(counts)]
+    p = A.placer_params(A.CONFIGS["stdcell_200k"], bins_x=128, bins_y=128, cell_order=1)
+    with BatchPlacer(nls, bins, p) as bp:
+        x0 = [bp.positions(i) for i in range(len(counts))]
+        info = bp.run()
+        res = bp.results()
+        xb = [bp.positions(i) for i in range(len(counts))]
+    assert info["finished"] == len(counts)
+    for i, (nl, b) in enumerate(zip(nls, bins)):
+        assert np.array_equal(x0[i][0][:nl.n_mov], nl.x[:nl.n_mov])  # caller's order, before the run
+        with A.Placer(nl, single_params(p, b)) as pl:
+            r = pl.run()
+            xs, ys = pl.get_positions()
+        assert res["iterations"][i] == r["iterations"] and res["evals"][i] == r["evals"], i
+        assert np.array_equal(xs, xb[i][0]) and np.array_equal(ys, xb[i][1]), i
+
+
 def test_forked_batch_matches_single():
     """The batch graph with its WA gradient on a second captured branch (the
     form the 64-instance workload takes while >= 400k cells run), forced here

@@ -122,6 +122,59 @@ def test_wa_gradient_and_hpwl(oracle, kind, n, bins):
                 assert abs(gx.sum()) <= 1e-3 * sc * np.sqrt(d.n_mov)
 
 
+def test_cell_order_keeps_the_callers_order(oracle):
+    """cell_order = 1 stores the movable cells in breadth-first order over the
+    netlist.  On a netlist stored in random order, every per-cell array that
+    crosses the boundary stays in the caller's order: positions, cell sizes,
+    bin ranges and pin cells exactly, the gradient components to FP32
+    summation order (a cell's pin partials are added in another order), and
+    both match the FP64 oracle built from the caller's netlist."""
+    say("case cell_order: shuffled 20k-cell netlist, 128^2 bins")
+    nl = A.generate("stdcell", seed=3, n_movable=20_000, shuffle_storage=1)
+    out = {}
+    for order in (0, 1):
+        p = A.placer_params(A.CONFIGS["stdcell_200k"], bins_x=128, bins_y=128, cell_order=order)
+        d, _, _ = oracle_design(oracle, nl, p)
+        with A.Placer(nl, p) as pl:
+            x, y = spread_positions(d, np.random.default_rng(8), 0.4)
+            pl.set_positions(x, y)
+            xs, ys = pl.get_positions()
+            assert np.array_equal(xs, x) and np.array_equal(ys, y)
+            bw = (nl.die[2] - nl.die[0]) / 128
+            g = pl.gradient(2.0 * bw, 3e-3)
+            bins = pl.cell_bins(True)
+            w, h, _, _ = pl.cells()
+            ns, pc, _, _, perm = pl.nets()
+        out[order] = (g, bins, w, h, ns, pc, perm)
+        o = d.gradient(x.astype(np.float64), y.astype(np.float64), p.target_density, 2.0 * bw, 3e-3)
+        for k in ("gx", "gy", "gwx", "gwy"):
+            assert np.abs(g[k] - o[k]).max() <= 2e-4 * np.abs(o[k]).max(), (order, k)
+    (g0, b0, w0, h0, ns0, pc0, pm0), (g1, b1, w1, h1, ns1, pc1, pm1) = out[0], out[1]
+    assert np.array_equal(b0, b1) and np.array_equal(w0, w1) and np.array_equal(h0, h1)
+    for k in ("gx", "gy", "gwx", "gwy", "fx", "fy"):
+        err = np.abs(g0[k] - g1[k]).max() / np.abs(g0[k]).max()
+        say(f"  {k}: order 0 vs 1 rel {err:.2e}")
+        assert err <= 1e-5, k
+    # every net holds the same cells, in the caller's numbering
+    def nets(ns, pc, pm):
+        return {int(pm[k]): sorted(pc[ns[k]:ns[k + 1]].tolist()) for k in range(len(pm))}
+    assert nets(ns0, pc0, pm0) == nets(ns1, pc1, pm1)
+
+
+def test_cell_order_run_converges():
+    say("case cell_order: full run, shuffled 20k cells")
+    nl = A.generate("stdcell", seed=3, n_movable=20_000, shuffle_storage=1)
+    res = {}
+    for order in (0, 1):
+        p = A.placer_params(A.CONFIGS["stdcell_200k"], bins_x=128, bins_y=128, cell_order=order, use_graph=1)
+        with A.Placer(nl, p) as pl:
+            res[order] = pl.run()
+    say(f"  order 0: it {res[0]['iterations']} hpwl {res[0]['hpwl']:.1f}; order 1: it {res[1]['iterations']} "
+        f"hpwl {res[1]['hpwl']:.1f}")
+    assert res[1]["overflow"] <= 0.10
+    assert res[1]["hpwl"] == pytest.approx(res[0]["hpwl"], rel=0.03)
+
+
 def test_wa_wide_and_degenerate_nets(oracle):
     """Nets wider than a wavefront (strided path), degree-1 and empty nets."""
     say("case wa wide/degenerate nets")
