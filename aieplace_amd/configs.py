@@ -2,7 +2,8 @@
 
 mesh_c1      : 576-cell 24x24 mesh, 32x32 bins, rho_t 1.0, gamma 0.5 bin widths (fixed)
 stdcell_200k : 200k cells, 512x512 bins, rho_t 0.9
-stdcell_2m   : 2M cells, 1024x1024 bins, rho_t 0.9 (the measured workload)
+stdcell_2m   : 2M cells, 1024x1024 bins, rho_t 0.9 (the measured workload); cells stored
+               in random order, as a LEF/DEF netlist may list its components
 batch_64     : 64 instances, 5k-150k cells, 128/256/512 bins
 learned_1m   : 1M cells, 1024x1024 bins (learned-step form)
 """
@@ -15,7 +16,7 @@ CONFIGS = {
                     gamma_schedule=0),
     "stdcell_200k": dict(gen=dict(kind="stdcell", n_movable=200_000), bins=512,
                          target_density=0.9, gamma_bins=4.0, gamma_schedule=1),
-    "stdcell_2m": dict(gen=dict(kind="stdcell", n_movable=2_000_000), bins=1024,
+    "stdcell_2m": dict(gen=dict(kind="stdcell", n_movable=2_000_000, shuffle_storage=1), bins=1024,
                        target_density=0.9, gamma_bins=4.0, gamma_schedule=1),
     "learned_1m": dict(gen=dict(kind="stdcell", n_movable=1_000_000), bins=1024,
                        target_density=0.9, gamma_bins=4.0, gamma_schedule=1),

@@ -444,15 +444,21 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     if (threadIdx.x < B)
         for (int k = 0; k < 10; ++k) sfin[threadIdx.x][k] = load_agent(&fin[threadIdx.x * 10 + k]);
     __syncthreads();
+    // thread k sums column k over the blocks in block order (ten short chains
+    // instead of one long one; the same order, so the same sums)
+    __shared__ double tot[10];
+    if (threadIdx.x < 10) {
+        const int k = threadIdx.x;
+        double t = sfin[0][k];
+        for (int j = 1; j < B; ++j) t = k < 9 ? t + sfin[j][k] : fmax(t, sfin[j][k]);
+        tot[k] = t;
+    }
+    __syncthreads();
     __shared__ int propose;
     __shared__ float feat[16];  // the decision thread's features, for the whole block
     if (threadIdx.x == 0) {
         double t[10];
-        for (int k = 0; k < 10; ++k) t[k] = sfin[0][k];
-        for (int j = 1; j < B; ++j) {
-            for (int k = 0; k < 9; ++k) t[k] += sfin[j][k];
-            t[9] = fmax(t[9], sfin[j][9]);
-        }
+        for (int k = 0; k < 10; ++k) t[k] = tot[k];
         __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // next launch
         DevResult R;
         R.hpwl = t[0];

@@ -355,7 +355,9 @@ def main():
                dtype="fp32" if head == "plain" else "bf16x3+fp32", data="synthetic",
                config=dict(workload=args.workload, movable_cells=nl.n_mov, fixed_cells=nl.n_fix,
                            nets=nl.n_nets, pins=nl.n_pins, bins=cfg["bins"],
-                           target_density=cfg["target_density"], headline_form=head),
+                           target_density=cfg["target_density"], headline_form=head,
+                           cell_storage=("random order, renumbered breadth-first by ep_create"
+                                         if cfg["gen"].get("shuffle_storage") else "generator order")),
                forms=results)
     if head != "plain":
         m = results[head]

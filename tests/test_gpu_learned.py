@@ -171,7 +171,11 @@ def test_learned_1m_workload():
     """The learned-step workload: 1M cells on 1024x1024 bins, split-BF16
     transforms, with and without the committed model (device-controlled runs).
     Reports acceptance, gradient evaluations per iteration and HPWL against the
-    same form without the model."""
+    same form without the model.  What the network gains is wirelength: its
+    run stops at the same overflow with at least 1% lower HPWL (it proposes
+    slightly shorter steps than the Lipschitz estimate, at the cost of a few
+    percent more evaluations).  Runs are deterministic, so the bound is a
+    fixed outcome, not a sample."""
     import os
     if not os.path.exists(learned.DEFAULT_WEIGHTS):
         pytest.skip("no committed step model")
@@ -192,8 +196,8 @@ def test_learned_1m_workload():
         f"{lrn['learned_proposals']} proposals, acceptance {rate:.3f}")
     assert plain["overflow"] <= 0.10 and lrn["overflow"] <= 0.10
     assert 0 < lrn["learned_proposals"] <= lrn["iterations"]
-    assert rate > 0.5
-    assert lrn["hpwl"] <= plain["hpwl"] * 1.005
+    assert rate > 0.9
+    assert lrn["hpwl"] <= plain["hpwl"] * 0.99
 
 
 def test_corrupted_model_falls_back():
