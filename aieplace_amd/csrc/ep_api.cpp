@@ -670,7 +670,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     EP_TRY(dalloc(&c->tA, 2 * nb));
     EP_TRY(dalloc(&c->tB, 2 * nb));
     EP_TRY(dalloc(&c->part, c->part_total));
-    EP_TRY(dalloc(&c->fin, 32 * 10));
+    EP_TRY(dalloc(&c->fin, kFinalizeBlocks * 10));
     EP_TRY(dalloc(&c->fin_ticket, 1));
     EP_TRY(dalloc(&c->d_scal, 1));
     EP_TRY(dalloc(&c->d_res, 1));

@@ -32,6 +32,9 @@
 namespace ep {
 
 constexpr int kBlock = 256;
+// Blocks of the final reduction (k_finalize); the context's fin buffer holds
+// ten partial sums per block.
+constexpr int kFinalizeBlocks = 32;
 // Launch caps of the reducing kernels (bounds the partial sums to finalize).
 constexpr int kWaGridCap = 16384;
 constexpr int kCellGridCap = 16384;

@@ -497,6 +497,20 @@ constexpr float kDensFix = (float)kDensFixUnits;
 
 __device__ inline unsigned long long dens_fix(float v) { return (unsigned long long)(v * kDensFix); }
 
+// Bin values from the accumulators, one rounding sequence for every kernel that
+// converts them (single and batched): the field map is
+// rho_t * fixed + smoothed cells, and a bin overflows by movable + rho_t *
+// fixed - rho_t * area (cap).
+__device__ inline float field_bin(float fx, unsigned long long af, double unit, float rho_t)
+{
+    return __fmaf_rn(rho_t, fx, (float)((double)af * unit));
+}
+__device__ inline float mov_bin(unsigned long long am, double unit) { return (float)((double)am * unit); }
+__device__ inline float ovf_bin(float mv, float fx, float rho_t, float cap)
+{
+    return __fsub_rn(__fmaf_rn(rho_t, fx, mv), cap);
+}
+
 // Adds the overlaps of rectangle r (in bin areas, scaled by sc) to plane
 // `plane` of the LDS window (x0, y0, nx, ny) when r lies inside it, else
 // straight into `acc`.  The window is the dynamic LDS block, declared here and

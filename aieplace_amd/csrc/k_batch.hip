@@ -247,11 +247,11 @@ __global__ __launch_bounds__(kBlock) void kb_overflow(BDev d)
         if (b < nb) {
             const size_t o = (size_t)I.map_off + b;
             const float fx = d.map_fix[o];
-            const float mv = (float)((double)d.acc_mov[o] * I.dens_unit);
-            const float fd = I.rho_t * fx + (float)((double)d.acc_field[o] * I.dens_unit);
+            const float mv = mov_bin(d.acc_mov[o], I.dens_unit);
+            const float fd = field_bin(fx, d.acc_field[o], I.dens_unit, I.rho_t);
             d.map_mov[o] = mv;
             d.map_field[o] = fd;
-            const float v = mv + I.rho_t * fx - I.cap;
+            const float v = ovf_bin(mv, fx, I.rho_t, I.cap);
             if (v > 0.0f) acc[0] += (double)v;
             const double f = fd;
             acc[1] += f;

@@ -168,11 +168,11 @@ __global__ __launch_bounds__(kBlock) void k_overflow(unsigned long long *__restr
         const unsigned long long am = acc_mov[b], af = acc_field[b];
         acc_mov[b] = 0ull;
         acc_field[b] = 0ull;
-        const float mv = (float)((double)am * unit);
-        const float fd = rho_t * fx + (float)((double)af * unit);
+        const float mv = mov_bin(am, unit);
+        const float fd = field_bin(fx, af, unit, rho_t);
         map_mov[b] = mv;
         map_field[b] = fd;
-        const float v = mv + rho_t * fx - cap;
+        const float v = ovf_bin(mv, fx, rho_t, cap);
         if (v > 0.0f) acc[0] += (double)v;
         const double f = fd;
         acc[1] += f;
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_pin_sum(const int32_t *__restrict__ 
 // writes its ten values to fin[b]; the last block to finish (atomic ticket)
 // sums fin[0..B) in block order, so the result does not depend on timing.
 constexpr int kFinalizeThreads = 256;
-constexpr int kFinalizeBlocks = 32;
+static_assert(kFinalizeBlocks <= kFinalizeThreads, "one loading thread per block result");
 
 struct StepCtlArgs {
     DevCtl *ctl;            // null: no step decision (host-controlled launches)
