@@ -86,32 +86,35 @@ def test_records_are_collected():
 
 def _ablation(w, seed, n, bins, precision=api.EP_PREC_BF16X3):
     """The same form with and without the model.  The final HPWL of a run
-    varies with the filler placement (and FP32 summation order), so the plain
-    form runs with filler seeds 7-10; `plain_hpwls` gives that spread."""
+    varies with the filler placement (and FP32 summation order), so both forms
+    run with filler seeds 7-10; `plain_hpwls` and `lrn_hpwls` give the spreads,
+    `plain` and `lrn` are the seed-7 runs."""
     nl, p = _instance(seed, n=n, bins=bins, use_graph=1, precision=precision)
-    plain_hpwls = []
-    for fs in (7, 8, 9, 10):
-        p.filler_seed = fs
-        with A.Placer(nl, p) as pl:
-            r = pl.run()
-        plain_hpwls.append(r["hpwl"])
-        if fs == 7:
-            plain = r
-    p.filler_seed = 7
-    p.learned_step = 1
-    with A.Placer(nl, p) as pl:
-        pl.set_step_model(w)
-        lrn = pl.run()
-    rate = lrn["learned_accepted"] / max(1, lrn["learned_proposals"])
+    runs = {0: [], 1: []}
+    for learned_step in (0, 1):
+        p.learned_step = learned_step
+        for fs in (7, 8, 9, 10):
+            p.filler_seed = fs
+            with A.Placer(nl, p) as pl:
+                if learned_step:
+                    pl.set_step_model(w)
+                r = pl.run()
+            runs[learned_step].append(r)
+            assert r["overflow"] <= 0.10
+            if learned_step:  # every iteration proposes unless the model sits out after a rejection
+                assert 0 < r["learned_proposals"] <= r["iterations"]
+    plain, lrn = runs[0][0], runs[1][0]
+    plain_hpwls = [r["hpwl"] for r in runs[0]]
+    lrn_hpwls = [r["hpwl"] for r in runs[1]]
+    rate = sum(r["learned_accepted"] for r in runs[1]) / max(1, sum(r["learned_proposals"] for r in runs[1]))
     say(f"  same form without model: {plain['iterations']} it {plain['evals']} evals "
         f"hpwl {plain['hpwl']:.1f} {plain['seconds']:.3f}s; with model: {lrn['iterations']} it "
         f"{lrn['evals']} evals hpwl {lrn['hpwl']:.1f} {lrn['seconds']:.3f}s; acceptance {rate:.3f} "
         f"({lrn['evals'] / lrn['iterations']:.3f} vs {plain['evals'] / plain['iterations']:.3f} evals/it); "
-        f"plain hpwl over filler seeds {min(plain_hpwls):.1f}..{max(plain_hpwls):.1f}")
-    assert lrn["overflow"] <= 0.10
-    # every iteration proposes unless the model sits out after a rejection (backoff)
-    assert 0 < lrn["learned_proposals"] <= lrn["iterations"]
-    return plain, lrn, rate, plain_hpwls
+        f"hpwl over filler seeds: plain {min(plain_hpwls):.1f}..{max(plain_hpwls):.1f} "
+        f"(mean {np.mean(plain_hpwls):.1f}), learned {min(lrn_hpwls):.1f}..{max(lrn_hpwls):.1f} "
+        f"(mean {np.mean(lrn_hpwls):.1f})")
+    return plain, lrn, rate, plain_hpwls, lrn_hpwls
 
 
 def test_learned_step_small_model(small_model):
@@ -119,9 +122,9 @@ def test_learned_step_small_model(small_model):
     fall back to the plain search, and the run still converges."""
     w, curve, nrec = small_model
     say(f"case learned step (4-run model, {nrec} records, loss {curve[0]:.3f} -> {curve[-1]:.3f})")
-    plain, lrn, rate, spread = _ablation(w, 31, 20_000, 128)
-    # within 0.5% of the plain form's filler-seed range
-    assert lrn["hpwl"] <= max(spread) * 1.005
+    plain, lrn, rate, spread, lspread = _ablation(w, 31, 20_000, 128)
+    # the mean over filler seeds within 0.5% of the plain form's range
+    assert np.mean(lspread) <= max(spread) * 1.005
 
 
 def test_learned_step_ablation_committed_model():
@@ -134,9 +137,9 @@ def test_learned_step_ablation_committed_model():
     w = learned.load()
     for seed, n, bins in ((31, 20_000, 128), (32, 200_000, 512)):
         say(f"case ablation seed={seed} n={n} bins={bins}")
-        plain, lrn, rate, spread = _ablation(w, seed, n, bins)
+        plain, lrn, rate, spread, lspread = _ablation(w, seed, n, bins)
         assert rate > 0.5
-        assert lrn["hpwl"] <= max(spread) * 1.005
+        assert np.mean(lspread) <= max(spread) * 1.005
 
 
 def test_learned_runs_start_fresh():
