@@ -19,6 +19,7 @@ EP_PREC_FP32, EP_PREC_BF16X3, EP_PREC_FP16X3 = 0, 1, 2
 EP_GEN_MESH, EP_GEN_STDCELL = 0, 1
 EP_TR_NONE, EP_TR_DCT, EP_TR_IDCT, EP_TR_IDXST = -1, 0, 1, 2
 EP_WA_PACKET_NOALLOC_MAX = 1 << 20
+EP_WA_EXP_HW, EP_WA_EXP_FAST64 = 0, 1
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
@@ -89,7 +90,7 @@ EXPORTS = [
     "ep_get_wa_grad", "ep_density", "ep_get_density", "ep_field", "ep_get_field", "ep_field_error", "ep_gradient",
     "ep_cell_bins", "ep_hpwl", "ep_overflow", "ep_init_optimizer", "ep_step", "ep_run",
     "ep_set_precision", "ep_profile", "ep_take_step_records", "ep_set_step_model", "ep_wa_packet",
-    "ep_transform", "ep_wa_packet_noalloc",
+    "ep_transform", "ep_wa_packet_noalloc", "ep_wa_packet_exp",
     "ep_gen_default_params", "ep_gen_create", "ep_gen_free",
     "ep_mlp_default_train_params", "ep_mlp_train", "ep_mlp_forward", "ep_mlp_grad",
     "ep_batch_create", "ep_batch_destroy", "ep_batch_reset", "ep_batch_init", "ep_batch_run",
@@ -174,6 +175,7 @@ def _load():
     L.ep_set_step_model.argtypes = [vp, _fp, C.c_int32]
     L.ep_wa_packet.argtypes = [_fp, C.c_int32, C.c_float, _fp, C.c_int32]
     L.ep_wa_packet_noalloc.argtypes = [_fp, C.c_int32, C.c_float, _fp, C.c_int32]
+    L.ep_wa_packet_exp.argtypes = [_fp, C.c_int32, C.c_float, _fp, C.c_int32, C.c_int32]
     L.ep_transform.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _fp, _fp, C.c_int32]
     L.ep_gen_default_params.argtypes = [C.POINTER(EpGenParams), C.c_int32]
     L.ep_gen_default_params.restype = None

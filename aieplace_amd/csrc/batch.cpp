@@ -64,6 +64,7 @@ struct ep_batch {
     hipStream_t side = nullptr;              // second captured branch (WA beside the maps and field)
     hipEvent_t fork = nullptr, join = nullptr;
     ep_params prm{};
+    bool f16() const { return prm.precision == EP_PREC_FP16X3; }  // products scale by the map range
     int n = 0;
     std::vector<BInst> inst;
     std::vector<int32_t> n_fix;
@@ -174,7 +175,7 @@ struct ep_batch {
         if (e != hipSuccess) return e;
         bl_wa(d, at, wa_blocks, side);
         if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
-        bl_dens(d, bin_blocks, dens_blocks[v], kDensCpt[v], s);
+        bl_dens(d, bin_blocks, dens_blocks[v], kDensCpt[v], f16(), s);
         launch_field_batch(bf, prm.precision, s);
         if ((e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
         bl_cell_grad(d, at, cell_blocks, s);
@@ -187,7 +188,7 @@ struct ep_batch {
     void eval(const BDev &d, int at, bool counted, hipStream_t s)
     {
         bl_maps(d, at, counted, bin_blocks, sort_blocks, dens_blocks[dens_variant],
-                kDensCpt[dens_variant], s);
+                kDensCpt[dens_variant], f16(), s);
         bl_wa(d, at, wa_blocks, s);
         launch_field_batch(bf, prm.precision, s);
         bl_cell_grad(d, at, cell_blocks, s);
@@ -311,10 +312,11 @@ static ep_status autotune(ep_batch *b)
     for (int v = 0; v < 3; ++v) {
         b->dens_variant = v;
         const BDev d = b->dev();
-        bl_maps(d, kAtCur, false, b->bin_blocks, b->sort_blocks, b->dens_blocks[v], kDensCpt[v], b->stream);
+        bl_maps(d, kAtCur, false, b->bin_blocks, b->sort_blocks, b->dens_blocks[v], kDensCpt[v], b->f16(),
+                b->stream);
         (void)hipEventRecord(e0, b->stream);
         for (int r = 0; r < kTuneReps; ++r)
-            bl_maps(d, kAtCur, false, b->bin_blocks, b->sort_blocks, b->dens_blocks[v], kDensCpt[v],
+            bl_maps(d, kAtCur, false, b->bin_blocks, b->sort_blocks, b->dens_blocks[v], kDensCpt[v], b->f16(),
                     b->stream);
         (void)hipEventRecord(e1, b->stream);
         hipError_t e = hipEventSynchronize(e1);
@@ -506,6 +508,12 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     EPB_TRY(dalloc(&b->map_mov, nb));
     EPB_TRY(dalloc(&b->acc_field, nb));
     EPB_TRY(dalloc(&b->acc_mov, nb));
+    // cleared once here; afterwards every overflow pass clears what its scatter added
+    if (hipMemset(b->acc_field, 0, sizeof(unsigned long long) * nb) != hipSuccess ||
+        hipMemset(b->acc_mov, 0, sizeof(unsigned long long) * nb) != hipSuccess) {
+        destroy_batch(b);
+        return fail(EP_EDEVICE, "ep_batch_create: hipMemset failed");
+    }
     EPB_TRY(dalloc(&b->map_fix, nb));
     EPB_TRY(dalloc(&b->exy, nb));
     EPB_TRY(dalloc(&b->tA, 2 * nb));
@@ -605,7 +613,7 @@ extern "C" ep_status ep_batch_init(ep_batch *b)
     hipStream_t s = b->stream;
     bl_move(d, kMoveInitU, b->sort_blocks, s);
     bl_maps(d, kAtCur, false, b->bin_blocks, b->sort_blocks, b->dens_blocks[b->dens_variant],
-            kDensCpt[b->dens_variant], s);
+            kDensCpt[b->dens_variant], b->f16(), s);
     bl_reduce(d, s);
     bl_control(d, kCtlGamma, s);
     // gradient pieces at v_0 with lambda = 0: |gW|_1 and |F|_1

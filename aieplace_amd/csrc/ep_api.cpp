@@ -1592,26 +1592,43 @@ static ep_status check_packet(const float *in, int32_t n_in, float gamma, const 
     return EP_OK;
 }
 
-extern "C" ep_status ep_wa_packet(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out)
+static ep_status wa_packet(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out, int32_t exp_mode,
+                           const char *who)
 {
+    if (exp_mode != EP_WA_EXP_HW && exp_mode != EP_WA_EXP_FAST64)
+        return fail(EP_EINVAL, "%s: unknown exponent mode %d", who, exp_mode);
     int net_size = 0, n_nets = 0, groups = 0;
     int64_t n_data = 0;
-    ep_status st = check_packet(in, n_in, gamma, out, n_out, "ep_wa_packet", &net_size, &n_nets, &groups, &n_data);
+    ep_status st = check_packet(in, n_in, gamma, out, n_out, who, &net_size, &n_nets, &groups, &n_data);
     if (st != EP_OK || n_data == 0) return st;
     float *d_in = nullptr, *d_out = nullptr;
     st = dalloc(&d_in, (size_t)n_data);
     if (st == EP_OK) st = dalloc(&d_out, (size_t)n_data);
     if (st == EP_OK && hipMemcpy(d_in, in + 4, sizeof(float) * n_data, hipMemcpyHostToDevice) != hipSuccess)
-        st = fail(EP_EDEVICE, "ep_wa_packet: upload failed");
+        st = fail(EP_EDEVICE, "%s: upload failed", who);
     if (st == EP_OK) {
-        launch_wa_packet(d_in, net_size, n_nets, groups, 1.0f / gamma, d_out, nullptr);
+        if (exp_mode == EP_WA_EXP_FAST64)
+            launch_wa_packet_fast64(d_in, net_size, n_nets, groups, (double)gamma, d_out, nullptr);
+        else
+            launch_wa_packet(d_in, net_size, n_nets, groups, 1.0f / gamma, d_out, nullptr);
         if ((st = check_launch()) == EP_OK &&
             hipMemcpy(out, d_out, sizeof(float) * n_data, hipMemcpyDeviceToHost) != hipSuccess)
-            st = fail(EP_EDEVICE, "ep_wa_packet: download failed");
+            st = fail(EP_EDEVICE, "%s: download failed", who);
     }
     if (d_in) (void)hipFree(d_in);
     if (d_out) (void)hipFree(d_out);
     return st;
+}
+
+extern "C" ep_status ep_wa_packet(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out)
+{
+    return wa_packet(in, n_in, gamma, out, n_out, EP_WA_EXP_HW, "ep_wa_packet");
+}
+
+extern "C" ep_status ep_wa_packet_exp(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out,
+                                      int32_t exp_mode)
+{
+    return wa_packet(in, n_in, gamma, out, n_out, exp_mode, "ep_wa_packet_exp");
 }
 
 extern "C" ep_status ep_wa_packet_noalloc(const float *in, int32_t n_in, float gamma, float *out, int32_t n_out)

@@ -112,12 +112,14 @@ enum BMoveMode { kMoveTrial = 0, kMoveProbe = 1, kMoveInitU = 2 };
 void bl_move(const BDev &d, int mode, int sort_blocks, hipStream_t s);
 // Density maps of every unfinished instance; `counted`: a kMoveTrial move made
 // the sort counts at the evaluated positions.
-// bl_maps = bl_sort (clear, tile sort) then bl_dens (scatter, overflow, map statistics).
+// bl_maps = bl_sort (tile sort) then bl_dens (scatter; overflow, which also
+// clears the accumulators; with `map_range`, the field-map range that the
+// split-FP16 products scale by).  The accumulators start cleared at creation.
 void bl_sort(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_cpt,
              hipStream_t s);
-void bl_dens(const BDev &d, int bin_blocks, int dens_blocks, int dens_cpt, hipStream_t s);
+void bl_dens(const BDev &d, int bin_blocks, int dens_blocks, int dens_cpt, bool map_range, hipStream_t s);
 void bl_maps(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_blocks,
-             int dens_cpt, hipStream_t s);
+             int dens_cpt, bool map_range, hipStream_t s);
 void bl_wa(const BDev &d, int at, int wa_blocks, hipStream_t s);
 void bl_cell_grad(const BDev &d, int at, int cell_blocks, hipStream_t s);
 void bl_reduce(const BDev &d, hipStream_t s);

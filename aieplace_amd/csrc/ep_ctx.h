@@ -236,6 +236,9 @@ void launch_probe(Ctx &c, const float2 *v, const float2 *g, float2 *v_new, hipSt
 void launch_mlp_propose(Ctx &c, hipStream_t s);
 void launch_wa_packet(const float *data, int net_size, int n_nets, int groups, float ginv,
                       float *out, hipStream_t s);
+// The packet's partials in FP64 with the AIE kernel's fast_exp (EP_WA_EXP_FAST64).
+void launch_wa_packet_fast64(const float *data, int net_size, int n_nets, int groups, double gamma, float *out,
+                             hipStream_t s);
 // Device address of the current GPU's fixed packet staging (2 x
 // EP_WA_PACKET_NOALLOC_MAX floats: data, then partials), looked up once per device.
 ep_status wa_packet_stage(float **stage);

@@ -111,25 +111,6 @@ __global__ __launch_bounds__(kBlock) void kb_move(BDev d, int mode)
 
 // ------------------------------------------------------------------ maps
 
-// Clears the instance's fixed-point scatter accumulators.
-__global__ __launch_bounds__(kBlock) void kb_fill(BDev d)
-{
-    const int i = find_inst(d.pre_bin, d.n_inst, blockIdx.x);
-    if (d.st[i].done) return;
-    const BInst &I = d.inst[i];
-    const int nb = I.M * I.M;
-    const int base = (blockIdx.x - d.pre_bin[i]) * kBinsPerBlock;
-#pragma unroll
-    for (int k = 0; k < kBinsPerBlock / kBlock; ++k) {
-        const int b = base + k * kBlock + threadIdx.x;
-        if (b < nb) {
-            const size_t o = (size_t)I.map_off + b;
-            d.acc_field[o] = 0ull;
-            d.acc_mov[o] = 0ull;
-        }
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void kb_sort_count(BDev d, int at)
 {
     __shared__ int hist[kSortTiles];
@@ -232,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void kb_density(BDev d)
 }
 
 // Accumulators -> FP32 maps, overflow numerator and field-map sum/max/-min per
-// block (as k_overflow).
+// block, clearing the accumulators for the next scatter (as k_overflow).
 __global__ __launch_bounds__(kBlock) void kb_overflow(BDev d)
 {
     const int i = find_inst(d.pre_bin, d.n_inst, blockIdx.x);
@@ -247,8 +228,11 @@ __global__ __launch_bounds__(kBlock) void kb_overflow(BDev d)
         if (b < nb) {
             const size_t o = (size_t)I.map_off + b;
             const float fx = d.map_fix[o];
-            const float mv = mov_bin(d.acc_mov[o], I.dens_unit);
-            const float fd = field_bin(fx, d.acc_field[o], I.dens_unit, I.rho_t);
+            const unsigned long long am = d.acc_mov[o], af = d.acc_field[o];
+            d.acc_mov[o] = 0ull;  // cleared for the next scatter
+            d.acc_field[o] = 0ull;
+            const float mv = mov_bin(am, I.dens_unit);
+            const float fd = field_bin(fx, af, I.dens_unit, I.rho_t);
             d.map_mov[o] = mv;
             d.map_field[o] = fd;
             const float v = ovf_bin(mv, fx, I.rho_t, I.cap);
@@ -543,13 +527,12 @@ void bl_move(const BDev &d, int mode, int sort_blocks, hipStream_t s)
 void bl_sort(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_cpt,
              hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_fill, dim3(bin_blocks), dim3(kBlock), 0, s, d);
     if (!counted) hipLaunchKernelGGL(kb_sort_count, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
     hipLaunchKernelGGL(kb_sort_scan, dim3(d.n_inst), dim3(kSortScanThreads), 0, s, d, kBlock * dens_cpt);
     hipLaunchKernelGGL(kb_sort_scatter, dim3(sort_blocks), dim3(kBlock), 0, s, d, at);
 }
 
-void bl_dens(const BDev &d, int bin_blocks, int dens_blocks, int dens_cpt, hipStream_t s)
+void bl_dens(const BDev &d, int bin_blocks, int dens_blocks, int dens_cpt, bool map_range, hipStream_t s)
 {
     const size_t lds = 2 * sizeof(unsigned long long) * (size_t)d.dens_wcap;
     if (dens_cpt == 2)
@@ -559,14 +542,14 @@ void bl_dens(const BDev &d, int bin_blocks, int dens_blocks, int dens_cpt, hipSt
     else
         hipLaunchKernelGGL(kb_density<4>, dim3(dens_blocks), dim3(kBlock), lds, s, d);
     hipLaunchKernelGGL(kb_overflow, dim3(bin_blocks), dim3(kBlock), 0, s, d);
-    hipLaunchKernelGGL(kb_map_stats, dim3(d.n_inst), dim3(kBlock), 0, s, d);
+    if (map_range) hipLaunchKernelGGL(kb_map_stats, dim3(d.n_inst), dim3(kBlock), 0, s, d);
 }
 
 void bl_maps(const BDev &d, int at, bool counted, int bin_blocks, int sort_blocks, int dens_blocks,
-             int dens_cpt, hipStream_t s)
+             int dens_cpt, bool map_range, hipStream_t s)
 {
     bl_sort(d, at, counted, bin_blocks, sort_blocks, dens_cpt, s);
-    bl_dens(d, bin_blocks, dens_blocks, dens_cpt, s);
+    bl_dens(d, bin_blocks, dens_blocks, dens_cpt, map_range, s);
 }
 
 void bl_wa(const BDev &d, int at, int wa_blocks, hipStream_t s)

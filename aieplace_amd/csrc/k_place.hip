@@ -302,6 +302,60 @@ __global__ void k_wa_packet(const float *__restrict__ data, int net_size, int n_
     }
 }
 
+// The same packet in FP64 with the AIE kernel's exponent, evaluated the way the
+// reference's golden generator does (python/AIEmath/computeTerm.py:6-56,
+// createGoldenPartials.py:48-70):
+//   fast_exp(a) = (1 + a / 2^16)^(2^16), 16 squarings (compute_abc.cpp:12-18);
+//   b and c summed over the pins in ascending order, which in the packet layout
+//   (pin 0 the maximum, the others ascending) is pins 1..n-1, then pin 0;
+//   partial = ((1 + x/g) b+ - c+/g) a+ / (b+ b+) - ((1 - x/g) b- + c-/g) a- / (b- b-).
+// IEEE double operations in that order without multiply-add contraction, so
+// for integer coordinates the doubles are the generator's, and the FP32
+// outputs are its values rounded once.
+__device__ inline double fast_exp64(double a)
+{
+#pragma clang fp contract(off)
+    double r = 1.0 + a / 65536.0;
+    for (int i = 0; i < 16; ++i) r *= r;
+    return r;
+}
+
+__global__ void k_wa_packet_fast64(const float *__restrict__ data, int net_size, int n_nets, int groups,
+                                   double gamma, float *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= groups * 8) return;
+    const int g = t >> 3, lane = t & 7;
+    const float *v = data + (size_t)g * net_size * 8 + lane;
+    float *o = out + (size_t)g * net_size * 8 + lane;
+    if (t >= n_nets) {
+        for (int p = 0; p < net_size; ++p) o[p * 8] = 0.0f;
+        return;
+    }
+    double xmx = (double)v[0], xmn = (double)v[0];
+    for (int p = 1; p < net_size; ++p) {
+        xmx = fmax(xmx, (double)v[p * 8]);
+        xmn = fmin(xmn, (double)v[p * 8]);
+    }
+    double bp = 0.0, cp = 0.0, bm = 0.0, cm = 0.0;
+    for (int k = 0; k < net_size; ++k) {
+        const double x = (double)v[(k + 1 < net_size ? k + 1 : 0) * 8];
+        const double ap = fast_exp64((x - xmx) / gamma), am = fast_exp64((xmn - x) / gamma);
+        bp += ap;
+        cp += x * ap;
+        bm += am;
+        cm += x * am;
+    }
+    for (int p = 0; p < net_size; ++p) {
+        const double x = (double)v[p * 8];
+        const double ap = fast_exp64((x - xmx) / gamma), am = fast_exp64((xmn - x) / gamma);
+        const double t1 = ((1.0 + x / gamma) * bp - cp / gamma) * ap / (bp * bp);
+        const double t2 = ((1.0 - x / gamma) * bm + cm / gamma) * am / (bm * bm);
+        o[p * 8] = (float)(t1 - t2);
+    }
+}
+
 // ------------------------------------------------------------------ per-cell gather
 
 // g = (gW - lambda * F) / max(1, pins + lambda * area); F = r * sum ov * E over
@@ -890,6 +944,14 @@ void launch_wa_packet(const float *data, int net_size, int n_nets, int groups, f
     const int threads = groups * 8;
     hipLaunchKernelGGL(k_wa_packet, dim3((threads + 255) / 256), dim3(256), 0, s, data, net_size,
                        n_nets, groups, ginv, out);
+}
+
+void launch_wa_packet_fast64(const float *data, int net_size, int n_nets, int groups, double gamma, float *out,
+                             hipStream_t s)
+{
+    const int threads = groups * 8;
+    hipLaunchKernelGGL(k_wa_packet_fast64, dim3((threads + 255) / 256), dim3(256), 0, s, data, net_size,
+                       n_nets, groups, gamma, out);
 }
 
 // Device staging of ep_wa_packet_noalloc: packet data in the first half, the

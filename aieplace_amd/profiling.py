@@ -125,6 +125,11 @@ def _rocprof(pre, cmd, out_dir, timeout):
     return full, res
 
 
+def _cmd_text(cmd) -> str:
+    """A command as the summaries print it: run from the repository root."""
+    return " ".join(cmd).replace(sys.executable, "python3").replace(ROOT + os.sep, "")
+
+
 def _csv(out_dir, suffix):
     files = glob.glob(os.path.join(out_dir, "**", f"*{suffix}"), recursive=True)
     if not files:
@@ -150,7 +155,7 @@ def stats(args) -> str:
     tot = sum(float(r["TotalDurationNs"]) for r in rows)
     prec = PREC_NAME[args.precision]
     lines = [f"# Kernel statistics: {args.workload}, {prec} transforms", "",
-             "Command (from the repository root):", "", "```", " ".join(full[:]).replace(sys.executable, "python3"),
+             "Command (from the repository root):", "", "```", _cmd_text(full),
              "```", "",
              f"One full placement (graph replays) followed by {args.reps} ep_profile repetitions of each "
              "kernel at the final placement.  In the placement the WA gradient and the per-cell pin sums "
@@ -223,7 +228,7 @@ def timeline(args) -> str:
             per.setdefault(k, []).append((off, dur))
     prec = PREC_NAME[args.precision]
     lines = [f"# Kernel timeline: {args.workload}, {prec} transforms", "",
-             "Command (from the repository root):", "", "```", " ".join(full).replace(sys.executable, "python3"),
+             "Command (from the repository root):", "", "```", _cmd_text(full),
              "```", "",
              f"One device-controlled placement ({len(trials)} trials), split at each Nesterov trial kernel; "
              f"the figures are means over the last {len(tail)} trials.", "",
@@ -248,7 +253,7 @@ def batch_stats(args) -> str:
     tot = sum(float(r["TotalDurationNs"]) for r in rows)
     prec = PREC_NAME[args.precision]
     lines = [f"# Kernel statistics: batched engine, {args.instances} instances, {prec} transforms", "",
-             "Command (from the repository root):", "", "```", " ".join(full).replace(sys.executable, "python3"),
+             "Command (from the repository root):", "", "```", _cmd_text(full),
              "```", "",
              "Two full batched runs (the first captures the graphs and autotunes the tiles; the second is the "
              "measured one), so every count is about twice one run's.  The WA gradient runs on a second captured "
@@ -334,7 +339,7 @@ def traffic(args) -> str:
              f"device-to-device hipMemcpy of {args.calib_mib} MiB (the runtime's copy kernel) to the bytes "
              "that copy moves.", "",
              "Commands (from the repository root):", ""]
-    lines += ["```"] + [" ".join(c).replace(sys.executable, "python3") for c in cmds] + ["```", ""]
+    lines += ["```"] + [_cmd_text(c) for c in cmds] + ["```", ""]
     lines += [f"Calibration: the copy of {copy_bytes} B reads FETCH_SIZE {fetch_copy:.0f} B and WRITE_SIZE "
               f"{write_copy:.0f} B: factors {f_fetch:.4f} (fetch) and {f_write:.4f} (write).", ""]
     lines += ["| kernel | FETCH KiB/dispatch | WRITE KiB/dispatch | scaled MB/dispatch (fetch+write) |",
@@ -408,7 +413,7 @@ def mfma(args) -> str:
              "each) and SQ_WAIT_INST_ANY over SQ_WAVE_CYCLES.", "",
              f"Written by `python -m aieplace_amd.profiling mfma --workload {args.workload} --precision "
              f"{args.precision}`, which runs (from the repository root):", "", "```",
-             " ".join(full).replace(sys.executable, "python3"), "```", "",
+             _cmd_text(full), "```", "",
              "| kernel | waves | MFMA instr / wave | MFMA busy | VALU instr / wave | VALU active | waiting |",
              "|---|---:|---:|---:|---:|---:|---:|"]
     busy_m = busy_c = 0.0

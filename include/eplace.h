@@ -215,6 +215,16 @@ ep_status ep_set_step_model(ep_ctx *ctx, const float *weights, int32_t n_weights
  * net_size*8 partials per group in the same lane order (compute_partials.cpp:11-63). */
 ep_status ep_wa_packet(const float *packet_in, int32_t n_in, float gamma, float *partials_out,
                        int32_t n_out);
+/* ep_wa_packet with a choice of exponent.  EP_WA_EXP_HW: the hardware exp in
+ * FP32, as ep_wa_packet.  EP_WA_EXP_FAST64: the AIE kernel's fast_exp,
+ * (1 + a/2^16)^(2^16) by 16 squarings (compute_abc.cpp:12-18), with the
+ * whole partial in FP64 in the order of the reference's golden generator
+ * (python/AIEmath/computeTerm.py:6-56, createGoldenPartials.py:48-70); for
+ * its packets (integer coordinates, pins after pin 0 ascending) the outputs
+ * are the golden partials rounded once to FP32. */
+enum { EP_WA_EXP_HW = 0, EP_WA_EXP_FAST64 = 1 };
+ep_status ep_wa_packet_exp(const float *packet_in, int32_t n_in, float gamma, float *partials_out,
+                           int32_t n_out, int32_t exp_mode);
 /* ep_wa_packet without any device allocation.  No counterpart in the reference:
  * for callers that cannot allocate device memory (tight packet loops, code beside
  * graph capture, harnesses that forbid hipMalloc).  The packet is staged in one

@@ -49,6 +49,29 @@ def test_wa_packet_golden_replay(oracle, bench):
     np.testing.assert_allclose(out, ref, rtol=1e-4, atol=1e-5)
 
 
+@pytest.mark.parametrize("bench", range(4))
+def test_wa_packet_fast_exp_golden_bit_exact(bench):
+    """EP_WA_EXP_FAST64: the AIE kernel's fast_exp with the partial in FP64, in
+    the order of the reference's golden generator.  Every output equals the
+    golden partial (an FP64 value) rounded once to FP32, bit for bit; the
+    hardware-exp mode of the same entry point equals ep_wa_packet."""
+    import ctypes as C
+    say(f"case wa packet fast_exp golden {bench}")
+    xin = np.ascontiguousarray(rd(f"x_in{bench}.dat"), np.float32)
+    gold = rd(f"partials{bench}.dat")
+    outs = {}
+    for mode in (api.EP_WA_EXP_HW, api.EP_WA_EXP_FAST64):
+        out = np.zeros(len(xin) - 4, np.float32)
+        rc = api.lib.ep_wa_packet_exp(api.fptr(xin), len(xin), C.c_float(4.0), api.fptr(out), len(out), mode)
+        assert rc == api.EP_OK
+        outs[mode] = out
+    np.testing.assert_array_equal(outs[api.EP_WA_EXP_FAST64], gold)
+    np.testing.assert_array_equal(outs[api.EP_WA_EXP_HW], packet(xin, 4.0)[1])
+    out = np.zeros(len(xin) - 4, np.float32)
+    assert api.lib.ep_wa_packet_exp(api.fptr(xin), len(xin), C.c_float(4.0), api.fptr(out), len(out), 7) \
+        == api.EP_EINVAL
+
+
 def test_wa_packet_wider_nets_and_padding(oracle):
     rng = np.random.default_rng(2)
     size, n = 7, 13  # two groups, 3 padded lanes
