@@ -172,11 +172,11 @@ __device__ inline float sym_store(const GemmArgs &a, int z, int h0, int n0, cons
 
 // ---------------------------------------------------------------- FP32
 //
-// SK half-k per step (one barrier pair per step; the next step's operands are
-// loaded while the MFMAs run).  The k order of the MFMA chain does not depend
-// on SK, so every SK gives bit-identical results; products on one map use long
-// steps, batched ones shorter steps so that two blocks share a CU.  The
-// staging lives in dynamic LDS (sizeof(StageF32<SK>) bytes per block).
+// SK half-k per step; the next step's operands are loaded while the MFMAs
+// run.  The k order of the MFMA chain does not depend on SK, so every SK gives
+// bit-identical results.  The single-instance products stage through two LDS
+// stages in turn (2 sizeof(StageF32<SK>) bytes of dynamic LDS per block), the
+// batched ones through one, between two barriers.
 
 template <int SK>
 struct StageF32 {
@@ -184,12 +184,13 @@ struct StageF32 {
     float b[2][SK][SBN + 4];     // [B+|B- or Be|Bo][k][column]
 };
 
-template <bool INV, int EPI, int SK>
+// PP: two LDS stages in turn, as in the split form (sym_split_tile).
+template <bool INV, int EPI, int SK, bool PP>
 __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
 {
     constexpr int P = SK / 16;  // load pieces per thread and step
     extern __shared__ __attribute__((aligned(16))) float gemm_lds[];
-    StageF32<SK> &S = *reinterpret_cast<StageF32<SK> *>(gemm_lds);
+    StageF32<SK> *St = reinterpret_cast<StageF32<SK> *>(gemm_lds);  // PP: St[0], St[1]
     const int N = a.N, K = a.K, Mh = a.M >> 1, Kh = K >> 1;
     const float *__restrict__ Ae = a.A[z];
     const float *__restrict__ Ao = a.Ao[z];
@@ -228,9 +229,7 @@ __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
             }
         }
     };
-    load(0);
-    for (int k0 = 0; k0 < Kh; k0 += SK) {
-        __syncthreads();
+    auto stage = [&](StageF32<SK> &S) {
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             const int ka = q * 16 + ak, kb = q * 16 + bk;
@@ -248,30 +247,54 @@ __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
                 S.b[1][kb][br] = d0 - m0; S.b[1][kb + 1][br] = d1 - m1;
             }
         }
-        __syncthreads();
-        if (k0 + SK < Kh) load(k0 + SK);  // next tiles in flight during the MFMAs
+    };
+    auto mma = [&](const StageF32<SK> &S) {
 #pragma unroll
         for (int kk = 0; kk < SK; kk += 2) {
             const float x = S.a[eo][kk + (lane >> 5)][wm + (lane & 31)];
             const float y = S.b[eo][kk + (lane >> 5)][lane & 31];
             acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x, y, acc, 0, 0, 0);
         }
+    };
+    if (PP) {
+        load(0);
+        stage(St[0]);
+        __syncthreads();
+        if (SK < Kh) load(SK);
+        int cur = 0;
+        for (int k0 = 0; k0 < Kh; k0 += SK) {
+            if (k0 + SK < Kh) stage(St[cur ^ 1]);
+            if (k0 + 2 * SK < Kh) load(k0 + 2 * SK);
+            mma(St[cur]);
+            __syncthreads();
+            cur ^= 1;
+        }
+    } else {
+        load(0);
+        for (int k0 = 0; k0 < Kh; k0 += SK) {
+            __syncthreads();
+            stage(St[0]);
+            __syncthreads();
+            if (k0 + SK < Kh) load(k0 + SK);  // next tiles in flight during the MFMAs
+            mma(St[0]);
+        }
     }
-    sym_store<INV, EPI>(a, z, h0, n0, acc, 1.0f, &S.a[0][0][0]);
+    sym_store<INV, EPI>(a, z, h0, n0, acc, 1.0f, &St[0].a[0][0][0]);
 }
 
 template <bool INV, int EPI, int SK>
 __global__ __launch_bounds__(256) void k_sym_f32(GemmArgs a)
 {
     if (a.skip && *a.skip) return;
-    sym_f32_tile<INV, EPI, SK>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
+    sym_f32_tile<INV, EPI, SK, true>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
 // ---------------------------------------------------------------- split 16-bit
 
 // SK half-k per step (SK / 16 sub-steps of three 32x32x16 MFMAs), staged in
-// dynamic LDS (sizeof(StageSplit<SK>) bytes per block); as in the FP32 form
-// the result does not depend on SK.
+// dynamic LDS (two sizeof(StageSplit<SK>) stages per block for the
+// single-instance products, one for the batched ones); as in the FP32 form the
+// result does not depend on SK.
 template <int SK>
 struct StageSplit {
     static constexpr int LD = SK + 8;  // padded LDS row (16-bit elements)
@@ -322,12 +345,18 @@ __device__ inline void split_store4(const float (&v)[4], uint16_t *dh, uint16_t 
     *reinterpret_cast<uint2 *>(dl) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
 }
 
-template <bool INV, int EPI, bool F16, int SK>
+// PP (single-map products): two LDS stages in turn.  While a step's MFMAs read
+// one stage, the same waves split the next step's data operand into the other
+// and issue the loads of the step after it, so the MFMA pipe works while the
+// VALU and LDS do; one barrier per step.  Without PP (batched products, two
+// blocks per CU): one stage, loaded between two barriers.  The MFMA chain's k
+// order is the same either way, so both give bit-identical products.
+template <bool INV, int EPI, bool F16, int SK, bool PP>
 __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
 {
     constexpr int P16 = SK / 32;  // load pieces per thread and step
     extern __shared__ __attribute__((aligned(16))) float gemm_lds[];
-    StageSplit<SK> &S = *reinterpret_cast<StageSplit<SK> *>(gemm_lds);
+    StageSplit<SK> *St = reinterpret_cast<StageSplit<SK> *>(gemm_lds);  // PP: St[0], St[1]
     const int N = a.N, K = a.K, Mh = a.M >> 1, Kh = K >> 1;
     const uint16_t *__restrict__ Aeh = a.Ah[z];
     const uint16_t *__restrict__ Ael = a.Al[z];
@@ -380,9 +409,9 @@ __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
             }
         }
     };
-    load(0);
-    for (int k0 = 0; k0 < Kh; k0 += SK) {
-        __syncthreads();
+    // the loaded registers -> one LDS stage (A planes as they are; the data
+    // operand through the forward butterfly and the hi/lo split)
+    auto stage = [&](StageSplit<SK> &S) {
 #pragma unroll
         for (int q = 0; q < P16; ++q) {
             const int ka = q * 32 + ak, kb = q * 32 + bk;
@@ -413,8 +442,8 @@ __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
             split_store4<F16>(p, &S.bh[0][br][kb], &S.bl[0][br][kb]);
             split_store4<F16>(m, &S.bh[1][br][kb], &S.bl[1][br][kb]);
         }
-        __syncthreads();
-        if (k0 + SK < Kh) load(k0 + SK);
+    };
+    auto mma = [&](const StageSplit<SK> &S) {
 #pragma unroll
         for (int ks = 0; ks < SK; ks += 16) {
             const int kk = ks + 8 * (lane >> 5);
@@ -424,9 +453,32 @@ __device__ inline void sym_split_tile(const GemmArgs &a, int z, int h0, int n0)
             acc = mfma16<F16>(xh, yl, acc);
             acc = mfma16<F16>(xl, yh, acc);
         }
+    };
+    if (PP) {
+        load(0);
+        stage(St[0]);
+        __syncthreads();
+        if (SK < Kh) load(SK);
+        int cur = 0;
+        for (int k0 = 0; k0 < Kh; k0 += SK) {
+            if (k0 + SK < Kh) stage(St[cur ^ 1]);          // the next step, beside this step's MFMAs
+            if (k0 + 2 * SK < Kh) load(k0 + 2 * SK);       // the step after it
+            mma(St[cur]);
+            __syncthreads();
+            cur ^= 1;
+        }
+    } else {
+        load(0);
+        for (int k0 = 0; k0 < Kh; k0 += SK) {
+            __syncthreads();
+            stage(St[0]);
+            __syncthreads();
+            if (k0 + SK < Kh) load(k0 + SK);
+            mma(St[0]);
+        }
     }
     const float m = sym_store<INV, EPI>(a, z, h0, n0, acc, 1.0f / bscale,
-                                        reinterpret_cast<float *>(&S.ah[0][0][0]));
+                                        reinterpret_cast<float *>(&St[0].ah[0][0][0]));
     if (F16 && a.cmax) track_max(a.cmax, m);
 }
 
@@ -434,7 +486,7 @@ template <bool INV, int EPI, bool F16, int SK>
 __global__ __launch_bounds__(256) void k_sym_split(GemmArgs a)
 {
     if (a.skip && *a.skip) return;
-    sym_split_tile<INV, EPI, F16, SK>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
+    sym_split_tile<INV, EPI, F16, SK, true>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
 // Batched products: block b runs tile tiles[b] = (instance, h0, n0, z) with the
@@ -446,7 +498,7 @@ __global__ __launch_bounds__(256) void k_sym_f32_b(const GemmArgs *__restrict__ 
     const int4 t = tiles[blockIdx.x];
     const GemmArgs &a = args[t.x];
     if (a.skip && *a.skip) return;
-    sym_f32_tile<INV, EPI, SK>(a, t.w, t.y, t.z);
+    sym_f32_tile<INV, EPI, SK, false>(a, t.w, t.y, t.z);
 }
 
 template <bool INV, int EPI, bool F16, int SK>
@@ -456,7 +508,7 @@ __global__ __launch_bounds__(256) void k_sym_split_b(const GemmArgs *__restrict_
     const int4 t = tiles[blockIdx.x];
     const GemmArgs &a = args[t.x];
     if (a.skip && *a.skip) return;
-    sym_split_tile<INV, EPI, F16, SK>(a, t.w, t.y, t.z);
+    sym_split_tile<INV, EPI, F16, SK, false>(a, t.w, t.y, t.z);
 }
 
 // ---------------------------------------------------------------- FP64 reference
@@ -553,12 +605,12 @@ __global__ __launch_bounds__(256) void k_field_err(const float2 *__restrict__ ex
 // K steps of the products: long steps for a single map (one block per CU
 // anyway), shorter ones when two maps run so that two blocks share a CU.
 constexpr int kSkF32Single = 32, kSkF32Batch = 32;
-constexpr int kSkSplitSingle = 128, kSkSplitBatch = 64;
+constexpr int kSkSplitSingle = 64, kSkSplitBatch = 32;  // two stages each: 110 KB (one map), 56 KB
 
 template <int SK>
 void gemm_f32(const GemmArgs &a, dim3 grid, bool inv, int epi, hipStream_t s)
 {
-    const size_t lds = sizeof(StageF32<SK>);
+    const size_t lds = 2 * sizeof(StageF32<SK>);  // two stages in turn
     if (inv) hipLaunchKernelGGL((k_sym_f32<true, 0, SK>), grid, dim3(256), lds, s, a);
     else if (epi) hipLaunchKernelGGL((k_sym_f32<false, 1, SK>), grid, dim3(256), lds, s, a);
     else hipLaunchKernelGGL((k_sym_f32<false, 0, SK>), grid, dim3(256), lds, s, a);
@@ -567,7 +619,7 @@ void gemm_f32(const GemmArgs &a, dim3 grid, bool inv, int epi, hipStream_t s)
 template <bool F16, int SK>
 void gemm_split(const GemmArgs &a, dim3 grid, bool inv, int epi, hipStream_t s)
 {
-    const size_t lds = sizeof(StageSplit<SK>);
+    const size_t lds = 2 * sizeof(StageSplit<SK>);  // two stages in turn
     if (inv) hipLaunchKernelGGL((k_sym_split<true, 0, F16, SK>), grid, dim3(256), lds, s, a);
     else if (epi) hipLaunchKernelGGL((k_sym_split<false, 1, F16, SK>), grid, dim3(256), lds, s, a);
     else hipLaunchKernelGGL((k_sym_split<false, 0, F16, SK>), grid, dim3(256), lds, s, a);
