@@ -72,6 +72,35 @@ def test_wa_packet_fast_exp_golden_bit_exact(bench):
         == api.EP_EINVAL
 
 
+def test_wa_packet_fast_exp_wider_nets(oracle):
+    """EP_WA_EXP_FAST64 on packets the golden set does not hold: nets of 3-9
+    pins with padded lanes, against the FP64 oracle's fast_exp partials (one
+    FP32 rounding apart)."""
+    import ctypes as C
+    rng = np.random.default_rng(5)
+    for size, n in ((3, 8), (5, 13), (9, 3)):
+        groups = (n + 7) // 8
+        vals = np.zeros((groups, size, 8), np.float32)
+        for g in range(groups):
+            for l in range(8):
+                if g * 8 + l < n:
+                    x = np.sort(rng.integers(0, 10_000, size)).astype(np.float32)
+                    vals[g, :, l] = np.concatenate([[x[-1]], x[:-1]])  # pin 0 = max, then ascending
+        inp = np.concatenate([[size, n, 0, 0], vals.ravel()]).astype(np.float32)
+        out = np.zeros(vals.size, np.float32)
+        rc = api.lib.ep_wa_packet_exp(api.fptr(inp), len(inp), C.c_float(4.0), api.fptr(out), len(out),
+                                      api.EP_WA_EXP_FAST64)
+        assert rc == api.EP_OK
+        out = out.reshape(groups, size, 8)
+        for g in range(groups):
+            for l in range(8):
+                if g * 8 + l < n:
+                    ref = oracle.wa_partials(vals[g, :, l].astype(np.float64), 4.0, fast=True)
+                    np.testing.assert_allclose(out[g, :, l], ref, rtol=2e-7, atol=1e-12)
+                else:
+                    assert np.all(out[g, :, l] == 0)
+
+
 def test_wa_packet_wider_nets_and_padding(oracle):
     rng = np.random.default_rng(2)
     size, n = 7, 13  # two groups, 3 padded lanes
