@@ -36,6 +36,7 @@ import os
 import re
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -472,7 +473,8 @@ def main(argv=None) -> int:
     ap.add_argument("--instances", type=int, default=64, help="batch: instances of the batched workload")
     ap.add_argument("--copy-mib", type=int, default=0, help="drive: copy calibration instead of placement")
     ap.add_argument("--calib-mib", type=int, default=256, help="traffic: size of the calibration copy")
-    ap.add_argument("--scratch", default=os.path.join(ROOT, "gpurun_out", "rocprof"))
+    ap.add_argument("--scratch", default=os.path.join(tempfile.gettempdir(), "eplace_rocprof"),
+                    help="where rocprofv3 writes its raw output (the summaries go to profiles/)")
     ap.add_argument("--timeout", type=int, default=300)
     ap.add_argument("--from-trace", action="store_true",
                     help="timeline: summarise the trace already under --scratch instead of tracing")
