@@ -276,6 +276,8 @@ static void destroy_ctx(ep_ctx *c)
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->h_res) (void)hipHostFree(c->h_res);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    for (hipEvent_t e : c->ctl_read)
+        if (e) (void)hipEventDestroy(e);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -623,6 +625,7 @@ extern "C" ep_status ep_create(const ep_netlist *nl, const ep_params *p, ep_ctx 
     if (he == hipSuccess) he = hipStreamCreate(&c->side);
     if (he == hipSuccess) he = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (he == hipSuccess) he = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+    for (int i = 0; i < 2 && he == hipSuccess; ++i) he = hipEventCreateWithFlags(&c->ctl_read[i], hipEventDisableTiming);
     if (he != hipSuccess) {
         destroy_ctx(c);
         return fail(EP_EDEVICE, "stream/event creation: %s", hipGetErrorString(he));
@@ -1209,8 +1212,10 @@ extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
 // proposal, trial, evaluation, finalize + the ep_step decision on the device).
 // The kernels take the current and trial buffers from Pair arguments indexed
 // by the device's `cur`, so the same graph serves every trial; the host
-// replays it kCtlReplays times per synchronisation and reads the control state
-// back.  After the stop the remaining trials return at once (halt flag).
+// replays it kCtlReplays times per batch and reads the control state back
+// after each batch.  Two batches are in flight: the host waits for the state
+// of one while the next one runs, so the GPU does not idle for the host's
+// round trip.  After the stop the remaining trials return at once (halt flag).
 
 constexpr int kCtlTrialsPerGraph = 4;
 constexpr int kCtlReplays = 2;
@@ -1247,7 +1252,7 @@ static ep_status ensure_ctl(ep_ctx *c)
     c->ctl_cap = 0;
     ep_status st;
     if (!c->d_ctl && (st = dalloc(&c->d_ctl, 1)) != EP_OK) return st;
-    if (!c->h_ctl && hipHostMalloc((void **)&c->h_ctl, sizeof(DevCtl)) != hipSuccess)
+    if (!c->h_ctl && hipHostMalloc((void **)&c->h_ctl, 2 * sizeof(DevCtl)) != hipSuccess)
         return fail(EP_ENOMEM, "ep_run: pinned control state allocation failed");
     if ((st = dalloc(&c->d_log, (size_t)cap)) != EP_OK) return st;
     if ((st = dalloc(&c->d_rec, (size_t)cap * 17)) != EP_OK) return st;
@@ -1341,20 +1346,30 @@ static ep_status run_device(ep_ctx *c, ep_step_info *log, int32_t log_cap, int *
     if (st != EP_OK) return st;
     const CtlParams p = ctl_params(c);
     if (!c->ctl_exec && (st = build_ctl_graph(c, p)) != EP_OK) return st;
-    DevCtl &H = *c->h_ctl;
-    ctl_upload_state(c, H);
+    ctl_upload_state(c, c->h_ctl[0]);
     EP_HIP(hipMemcpyAsync(c->d_ctl, c->h_ctl, sizeof(DevCtl), hipMemcpyHostToDevice, c->stream));
     c->h_scal->sel = c->cur;
     c->h_scal->halt = 0;
     push_scalars(*c, c->stream);
     launch_step_begin(*c, p, c->stream);
     if ((st = check_launch()) != EP_OK) return st;
-    for (;;) {
+    // batch b reads the state back into slot b % 2; the state upload above has
+    // left slot 0 before batch 0's read-back is queued behind it on the stream
+    int slot = 0;
+    for (int64_t b = 0;; ++b, slot ^= 1) {
         for (int r = 0; r < kCtlReplays; ++r) EP_HIP(hipGraphLaunch(c->ctl_exec, c->stream));
-        EP_HIP(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
-        if ((st = sync(*c)) != EP_OK) return st;
-        if (H.done || H.error) break;
+        EP_HIP(hipMemcpyAsync(c->h_ctl + slot, c->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
+        EP_HIP(hipEventRecord(c->ctl_read[slot], c->stream));
+        if (b == 0) continue;
+        const hipError_t e = hipEventSynchronize(c->ctl_read[slot ^ 1]);
+        if (e != hipSuccess) return fail(EP_EDEVICE, "ep_run: %s", hipGetErrorString(e));
+        const DevCtl &P = c->h_ctl[slot ^ 1];
+        if (P.done || P.error) break;
     }
+    // the batch queued after the stop returned at once: the last read-back
+    // (slot) holds the final state
+    if ((st = sync(*c)) != EP_OK) return st;
+    DevCtl &H = c->h_ctl[slot];
     const int n = std::min(H.n_log, c->ctl_cap);
     *n_iter = n;
     if (log && log_cap > 0 && n > 0)

@@ -319,7 +319,8 @@ struct Ctx {
     bool graph_built[2] = {false, false};
     // device-controlled runs: one trial + step decision per replay
     hipGraphExec_t ctl_exec = nullptr;
-    DevCtl *d_ctl = nullptr, *h_ctl = nullptr;   // device / pinned host
+    DevCtl *d_ctl = nullptr, *h_ctl = nullptr;   // device / pinned host (two read-back slots)
+    hipEvent_t ctl_read[2] = {nullptr, nullptr};  // each slot's read-back done
     ep_step_info *d_log = nullptr;               // accepted iterations of the run
     float *d_rec = nullptr;                      // step records: 17 floats each
     int32_t ctl_cap = 0;                         // entries of d_log and d_rec

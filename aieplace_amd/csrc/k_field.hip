@@ -46,6 +46,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 constexpr float kF16Top = 14.0f;  // scaled data max lies in [2^14, 2^15)
@@ -304,13 +305,6 @@ struct StageSplit {
     uint16_t bh[2][SBN][LD], bl[2][SBN][LD];  // [B+|B- or Be|Bo][column][k]
 };
 
-__device__ inline void split_f16(float x, uint16_t &h, uint16_t &l)
-{
-    const _Float16 hh = (_Float16)x;
-    const _Float16 ll = (_Float16)(x - (float)hh);
-    h = __builtin_bit_cast(uint16_t, hh);
-    l = __builtin_bit_cast(uint16_t, ll);
-}
 
 template <bool F16>
 __device__ inline f32x16 mfma16(const uint16_t *pa, const uint16_t *pb, f32x16 acc)
@@ -322,36 +316,35 @@ __device__ inline f32x16 mfma16(const uint16_t *pa, const uint16_t *pb, f32x16 a
                                                   *reinterpret_cast<const bf16x8 *>(pb), acc, 0, 0, 0);
 }
 
-// Two values rounded to BF16 (nearest even) in one instruction: x0 in the low
-// half, x1 in the high half; for finite values the same bits as rounding the
-// FP32 pattern (u + 0x7FFF + bit 16 of u) >> 16, as the host's bf16_rne_host.
-__device__ inline uint32_t bf16_rne2(float x0, float x1)
+// Splits two values into 16-bit hi and lo parts, x0 in the low half of each
+// word.  Both conversions are the packed hardware ones (one instruction for two
+// values, rounding to nearest even); for finite values the BF16 bits equal
+// rounding the FP32 pattern, (u + 0x7FFF + bit 16 of u) >> 16, as the host's
+// bf16_rne_host does for the constant matrices.
+template <bool F16>
+__device__ inline void split2(float x0, float x1, uint32_t &h, uint32_t &l)
 {
     const f32x2 x = {x0, x1};
-    return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+    if (F16) {
+        const f16x2 hh = __builtin_convertvector(x, f16x2);
+        h = __builtin_bit_cast(uint32_t, hh);
+        l = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - __builtin_convertvector(hh, f32x2), f16x2));
+    } else {
+        h = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+        const f32x2 hf = {__uint_as_float(h << 16), __uint_as_float(h & 0xFFFF0000u)};
+        l = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - hf, bf16x2));
+    }
 }
 
 // Splits four values into hi and lo planes and stores them as 4 x 16 bits each.
 template <bool F16>
 __device__ inline void split_store4(const float (&v)[4], uint16_t *dh, uint16_t *dl)
 {
-    if (!F16) {
-        uint32_t h[2], l[2];
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            h[e] = bf16_rne2(v[2 * e], v[2 * e + 1]);
-            l[e] = bf16_rne2(v[2 * e] - __uint_as_float(h[e] << 16),
-                             v[2 * e + 1] - __uint_as_float(h[e] & 0xFFFF0000u));
-        }
-        *reinterpret_cast<uint2 *>(dh) = make_uint2(h[0], h[1]);
-        *reinterpret_cast<uint2 *>(dl) = make_uint2(l[0], l[1]);
-        return;
-    }
-    uint16_t h[4], l[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) split_f16(v[e], h[e], l[e]);
-    *reinterpret_cast<uint2 *>(dh) = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
-    *reinterpret_cast<uint2 *>(dl) = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
+    uint32_t h[2], l[2];
+    split2<F16>(v[0], v[1], h[0], l[0]);
+    split2<F16>(v[2], v[3], h[1], l[1]);
+    *reinterpret_cast<uint2 *>(dh) = make_uint2(h[0], h[1]);
+    *reinterpret_cast<uint2 *>(dl) = make_uint2(l[0], l[1]);
 }
 
 // PP (single-map products): two LDS stages in turn.  While a step's MFMAs read

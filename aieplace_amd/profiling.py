@@ -220,6 +220,11 @@ def timeline(args) -> str:
     tail = [t for t in trials[len(trials) // 2:] if t[0] > 0.3 * med]
     wall = sum(t[0] for t in tail) / len(tail) / 1e3
     busy = sum(t[1] for t in tail) / len(tail) / 1e3
+    # trials that end with the host's read-back of the control state (a runtime
+    # copy kernel) against the others: the difference is what the host's
+    # round trip between two batches of graph replays costs the GPU
+    rb = [t[0] for t in tail if any(n.startswith("__amd_rocclr_copy") for n, _, _ in t[2])]
+    other = [t[0] for t in tail if not any(n.startswith("__amd_rocclr_copy") for n, _, _ in t[2])]
     per = {}
     for _, _, ks in tail:
         seen = {}
@@ -234,8 +239,11 @@ def timeline(args) -> str:
              f"One device-controlled placement ({len(trials)} trials), split at each Nesterov trial kernel; "
              f"the figures are means over the last {len(tail)} trials.", "",
              f"Trial wall time {wall:.1f} us; kernels busy (union of both captured branches) {busy:.1f} us; "
-             f"idle between kernels {wall - busy:.1f} us ({100 * (wall - busy) / wall:.0f}%).", "",
-             "| kernel | start offset us | duration us |", "|---|---:|---:|"]
+             f"idle between kernels {wall - busy:.1f} us ({100 * (wall - busy) / wall:.0f}%).", ""]
+    if rb and other:
+        lines += [f"Trials ending with the control-state read-back: {len(rb)}, mean wall "
+                  f"{sum(rb) / len(rb) / 1e3:.1f} us; the other {len(other)}: {sum(other) / len(other) / 1e3:.1f} us.", ""]
+    lines += ["| kernel | start offset us | duration us |", "|---|---:|---:|"]
     for k, v in sorted(per.items(), key=lambda kv: sum(o for o, _ in kv[1]) / len(kv[1])):
         lines.append(f"| {k} | {sum(o for o, _ in v) / len(v) / 1e3:.1f} | {sum(d for _, d in v) / len(v) / 1e3:.1f} |")
     lines.append(f"\ntimeline wall_us {wall:.2f} busy_us {busy:.2f}")
