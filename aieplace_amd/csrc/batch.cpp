@@ -100,7 +100,9 @@ struct ep_batch {
     std::vector<char> active;
     int dens_variant = 1;
     BatchField *bf = nullptr;
-    BState *h_st = nullptr;  // pinned
+    BState *h_st = nullptr;  // pinned: the last control state the host has read
+    BState *h_rb[2] = {};     // pinned: read-back slots of two queued replays
+    hipEvent_t rb_done[2] = {};
     hipGraphExec_t exec = nullptr;
     bool ready = false;
 
@@ -210,6 +212,10 @@ static void destroy_batch(ep_batch *b)
         if (p) (void)hipFree(p);
     batch_field_free(b->bf);
     if (b->h_st) (void)hipHostFree(b->h_st);
+    for (int k = 0; k < 2; ++k) {
+        if (b->h_rb[k]) (void)hipHostFree(b->h_rb[k]);
+        if (b->rb_done[k]) (void)hipEventDestroy(b->rb_done[k]);
+    }
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->join) (void)hipEventDestroy(b->join);
     if (b->side) (void)hipStreamDestroy(b->side);
@@ -484,7 +490,9 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     (void)hipGetDevice(&b->device);
     if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->side) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->rb_done[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->rb_done[1], hipEventDisableTiming) != hipSuccess) {
         destroy_batch(b);
         return fail(EP_EDEVICE, "ep_batch_create: stream/event creation failed");
     }
@@ -533,7 +541,9 @@ extern "C" ep_status ep_batch_create(const ep_netlist *const *nls, const int32_t
     EPB_TRY(dalloc(&b->part_wa, 2 * (size_t)std::max(1, b->wa_blocks)));
     EPB_TRY(dalloc(&b->part_ovf, 4 * (size_t)b->bin_blocks));
     EPB_TRY(dalloc(&b->part_cell, 7 * (size_t)b->cell_blocks));
-    if (hipHostMalloc((void **)&b->h_st, sizeof(BState) * n, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&b->h_st, sizeof(BState) * n, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_rb[0], sizeof(BState) * n, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_rb[1], sizeof(BState) * n, hipHostMallocDefault) != hipSuccess) {
         destroy_batch(b);
         return fail(EP_ENOMEM, "ep_batch_create: pinned host allocation failed");
     }
@@ -639,7 +649,7 @@ extern "C" ep_status ep_batch_init(ep_batch *b)
 }
 
 // The graph holds device work only (kernels, the fork/join events): the host
-// reads the control state with its own copy after each replay.
+// reads the control state with its own copy after each replay (ep_batch_run).
 static ep_status build_graph(ep_batch *b)
 {
     const BDev d = b->dev();
@@ -686,20 +696,48 @@ extern "C" ep_status ep_batch_run(ep_batch *b, int32_t max_launches, ep_batch_ru
         return true;
     };
     auto count = [](const std::vector<char> &a) { return (int)std::count(a.begin(), a.end(), 1); };
-    while (launches < max_launches && !all_done()) {
-        EPB_HIP(hipGraphLaunch(b->exec, b->stream));
-        EPB_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(BState) * b->n, hipMemcpyDeviceToHost, b->stream));
-        EPB_HIP(hipStreamSynchronize(b->stream));
-        ++launches;
-        if ((st = check_errors(b)) != EP_OK) return st;
-        // once half of the instances in the launches have finished, drop them
-        std::vector<char> running(b->n, 0);
-        for (int i = 0; i < b->n; ++i) running[i] = b->active[i] && !b->h_st[i].done;
-        const int r = count(running);
-        if (r > 0 && 2 * r <= count(b->active)) {
-            if ((st = set_tables(b, running)) != EP_OK || (st = build_graph(b)) != EP_OK) return st;
+    const size_t st_bytes = sizeof(BState) * b->n;
+    // Two replays in flight: each replay's control state is read back into
+    // slot h_rb[k], and the host waits for one replay's state while the next
+    // replay runs.  Decisions therefore see the state one replay late: a
+    // replay queued after every instance has stopped returns at once (stopped
+    // instances skip their work), and the block tables are rebuilt only after
+    // the stream has drained.
+    int slot = 0, last = -1;
+    bool pending = false;  // h_rb[slot ^ 1] holds a queued read-back not yet checked
+    for (;;) {
+        const bool launch = launches < max_launches && !all_done();
+        if (launch) {
+            EPB_HIP(hipGraphLaunch(b->exec, b->stream));
+            EPB_HIP(hipMemcpyAsync(b->h_rb[slot], b->d_st, st_bytes, hipMemcpyDeviceToHost, b->stream));
+            EPB_HIP(hipEventRecord(b->rb_done[slot], b->stream));
+            ++launches;
+            last = slot;
         }
+        if (pending) {
+            EPB_HIP(hipEventSynchronize(b->rb_done[slot ^ 1]));
+            std::memcpy(b->h_st, b->h_rb[slot ^ 1], st_bytes);
+            if ((st = check_errors(b)) != EP_OK) {
+                (void)hipStreamSynchronize(b->stream);
+                return st;
+            }
+            if (all_done()) break;
+            // once half of the instances in the launches have finished, drop them
+            std::vector<char> running(b->n, 0);
+            for (int i = 0; i < b->n; ++i) running[i] = b->active[i] && !b->h_st[i].done;
+            const int r = count(running);
+            if (r > 0 && 2 * r <= count(b->active)) {
+                EPB_HIP(hipStreamSynchronize(b->stream));  // the queued replay reads the tables
+                if ((st = set_tables(b, running)) != EP_OK || (st = build_graph(b)) != EP_OK) return st;
+            }
+        }
+        if (!launch) break;
+        pending = true;
+        slot ^= 1;
     }
+    EPB_HIP(hipStreamSynchronize(b->stream));
+    if (last >= 0) std::memcpy(b->h_st, b->h_rb[last], st_bytes);  // the last replay's state
+    if ((st = check_errors(b)) != EP_OK) return st;
     const auto t1 = std::chrono::steady_clock::now();
     if (info) {
         info->instances = b->n;
