@@ -441,6 +441,8 @@ __global__ __launch_bounds__(kBlock) void k_pin_sum(const int32_t *__restrict__ 
 // sums fin[0..B) in block order, so the result does not depend on timing.
 constexpr int kFinalizeThreads = 256;
 static_assert(kFinalizeBlocks <= kFinalizeThreads, "one loading thread per block result");
+constexpr int kCtlWords = (int)(sizeof(DevCtl) / 4);
+static_assert(sizeof(DevCtl) % 4 == 0 && kCtlWords <= kFinalizeThreads, "one word of the control state per thread");
 
 struct StepCtlArgs {
     DevCtl *ctl;            // null: no step decision (host-controlled launches)
@@ -450,7 +452,7 @@ struct StepCtlArgs {
     const float *mlp_w;     // learned step: the model proposes the next trial's step
 };
 
-__device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
+__device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevCtl &S0, const DevResult &R,
                                    DevScalars *__restrict__ sc, const CtlParams &p,
                                    ep_step_info *__restrict__ log, float *__restrict__ rec);
 
@@ -462,6 +464,15 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
 {
     if (sc->halt) return;
     const int B = gridDim.x, b = blockIdx.x;
+    // Device-controlled runs: the control state and the trial's step are
+    // loaded now, beside the partial sums, so that the decision at the end of
+    // the last block does not wait for them (the previous launch wrote them).
+    __shared__ DevCtl ctl_s;
+    __shared__ float alpha_s;
+    uint32_t ctl_w = 0;
+    const bool ctl_load = ctl.ctl && threadIdx.x < kCtlWords;
+    if (ctl_load) ctl_w = reinterpret_cast<const uint32_t *>(ctl.ctl)[threadIdx.x];
+    const float alpha_used = sc->alpha;
     auto slice = [&](int n, int &lo, int &hi) {
         lo = (int)((int64_t)n * b / B);
         hi = (int)((int64_t)n * (b + 1) / B);
@@ -485,7 +496,9 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     for (int i = lo + threadIdx.x; i < hi; i += kFinalizeThreads) v[3] += part[trial_off + i];
     __shared__ double out[10];
     __shared__ int last;
-    block_reduce<10, 1, kFinalizeThreads>(v, out);
+    if (ctl_load) reinterpret_cast<uint32_t *>(&ctl_s)[threadIdx.x] = ctl_w;
+    if (threadIdx.x == 0) alpha_s = alpha_used;
+    block_reduce<10, 1, kFinalizeThreads>(v, out);  // its barriers also publish ctl_s and alpha_s
     if (threadIdx.x == 0) {
         for (int k = 0; k < 10; ++k) fin[b * 10 + k] = out[k];
         __threadfence();
@@ -526,9 +539,9 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
         R.gnorm2 = t[7];
         R.nonfinite = (int64_t)t[8];
         R.gmax = t[9];
-        R.alpha_used = sc->alpha;
+        R.alpha_used = alpha_s;
         *res = R;
-        if (ctl.ctl) step_decide(ctl.ctl, R, sc, ctl.p, ctl.log, ctl.rec);  // device-controlled run
+        if (ctl.ctl) step_decide(ctl.ctl, ctl_s, R, sc, ctl.p, ctl.log, ctl.rec);  // device-controlled run
         propose = ctl.mlp_w && sc->use_model && !sc->halt;
         if (propose)
             for (int k = 0; k < 16; ++k) feat[k] = sc->feat[k];
@@ -683,13 +696,14 @@ __device__ inline void ctl_begin(DevCtl &S, DevScalars *sc, const CtlParams &p)
     sc->use_model = propose ? 1 : 0;
 }
 
-// The decision after a trial whose reductions are R (one thread).
-__device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevResult &R,
+// The decision after a trial whose reductions are R (one thread); S0 is the
+// control state *ctl as the trial left it, the result goes back to *ctl.
+__device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevCtl &S0, const DevResult &R,
                                    DevScalars *__restrict__ sc, const CtlParams &p,
                                    ep_step_info *__restrict__ log, float *__restrict__ rec)
 {
 #pragma clang fp contract(off)
-    DevCtl S = *ctl;
+    DevCtl S = S0;
     if (S.done) return;
     S.evals++;
     S.trials++;
