@@ -442,6 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_pin_sum(const int32_t *__restrict__ 
 constexpr int kFinalizeThreads = 256;
 static_assert(kFinalizeBlocks <= kFinalizeThreads, "one loading thread per block result");
 constexpr int kCtlWords = (int)(sizeof(DevCtl) / 4);
+constexpr int kMlpWordsPerThread = (EP_MLP_NWEIGHTS + kFinalizeThreads - 1) / kFinalizeThreads;
 static_assert(sizeof(DevCtl) % 4 == 0 && kCtlWords <= kFinalizeThreads, "one word of the control state per thread");
 
 struct StepCtlArgs {
@@ -473,6 +474,16 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     const bool ctl_load = ctl.ctl && threadIdx.x < kCtlWords;
     if (ctl_load) ctl_w = reinterpret_cast<const uint32_t *>(ctl.ctl)[threadIdx.x];
     const float alpha_used = sc->alpha;
+    // The step model's weights likewise (learned runs): registers now, LDS
+    // after the partial sums, read by the proposal at the end.
+    __shared__ float mlp_s[EP_MLP_NWEIGHTS];
+    float mlp_r[kMlpWordsPerThread];
+    if (ctl.mlp_w)
+#pragma unroll
+        for (int k = 0; k < kMlpWordsPerThread; ++k) {
+            const int i = threadIdx.x + k * kFinalizeThreads;
+            mlp_r[k] = i < EP_MLP_NWEIGHTS ? ctl.mlp_w[i] : 0.0f;
+        }
     auto slice = [&](int n, int &lo, int &hi) {
         lo = (int)((int64_t)n * b / B);
         hi = (int)((int64_t)n * (b + 1) / B);
@@ -498,7 +509,13 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     __shared__ int last;
     if (ctl_load) reinterpret_cast<uint32_t *>(&ctl_s)[threadIdx.x] = ctl_w;
     if (threadIdx.x == 0) alpha_s = alpha_used;
-    block_reduce<10, 1, kFinalizeThreads>(v, out);  // its barriers also publish ctl_s and alpha_s
+    if (ctl.mlp_w)
+#pragma unroll
+        for (int k = 0; k < kMlpWordsPerThread; ++k) {
+            const int i = threadIdx.x + k * kFinalizeThreads;
+            if (i < EP_MLP_NWEIGHTS) mlp_s[i] = mlp_r[k];
+        }
+    block_reduce<10, 1, kFinalizeThreads>(v, out);  // its barriers also publish ctl_s, alpha_s, mlp_s
     if (threadIdx.x == 0) {
         for (int k = 0; k < 10; ++k) fin[b * 10 + k] = out[k];
         __threadfence();
@@ -549,7 +566,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     // the model's step for the next trial, while this block is still resident
     if (ctl.mlp_w) {
         __syncthreads();
-        mlp_propose_block(ctl.mlp_w, feat, sc, propose != 0);
+        mlp_propose_block(mlp_s, feat, sc, propose != 0);
     }
 }
 
