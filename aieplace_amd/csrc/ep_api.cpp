@@ -1276,16 +1276,16 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
         launch_trial(x, vc, uc, gc, vn, un, true, s);
         launch_sort(x, vt, true, s);
         if (x.n_place >= fork_min_cells(x.prm)) {
-            // fork: the WA gradient and the per-cell pin sums beside the density
-            // scatter and the transform products
+            // fork: the density scatter and the transform products on the second
+            // stream, the WA gradient and the per-cell pin sums beside them
             EP_CAP(hipEventRecord(x.fork, s), s);
             EP_CAP(hipStreamWaitEvent(x.side, x.fork, 0), s);
-            launch_wa(x, vt, x.side);
-            launch_pin_sum(x, x.side);
+            launch_density(x, x.side);
+            launch_overflow(x, x.side);
+            launch_field(x, x.prm.precision, x.side);
             EP_CAP(hipEventRecord(x.join, x.side), s);
-            launch_density(x, s);
-            launch_overflow(x, s);
-            launch_field(x, x.prm.precision, s);
+            launch_wa(x, vt, s);
+            launch_pin_sum(x, s);
             EP_CAP(hipStreamWaitEvent(s, x.join, 0), s);
             launch_cell_grad(x, vt, none, gc, gn, nullptr, nullptr, s, x.gw_cell);
         } else {
