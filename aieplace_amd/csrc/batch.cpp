@@ -166,8 +166,9 @@ struct ep_batch {
         return n_cells;
     }
 
-    // One gradient evaluation as a captured fork: the WA gradient on the side
-    // stream beside the density scatter and the transform products.
+    // One gradient evaluation as a captured fork: the density scatter and the
+    // transform products on the side stream beside the WA gradient (the
+    // captured stream of the longer branch decides how soon it starts).
     hipError_t eval_forked(const BDev &d, int at, bool counted, hipStream_t s)
     {
         const int v = dens_variant;
@@ -175,10 +176,10 @@ struct ep_batch {
         hipError_t e = hipEventRecord(fork, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
         if (e != hipSuccess) return e;
-        bl_wa(d, at, wa_blocks, side);
+        bl_dens(d, bin_blocks, dens_blocks[v], kDensCpt[v], f16(), side);
+        launch_field_batch(bf, prm.precision, side);
         if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
-        bl_dens(d, bin_blocks, dens_blocks[v], kDensCpt[v], f16(), s);
-        launch_field_batch(bf, prm.precision, s);
+        bl_wa(d, at, wa_blocks, s);
         if ((e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
         bl_cell_grad(d, at, cell_blocks, s);
         bl_reduce(d, s);
