@@ -513,7 +513,9 @@ __device__ inline float ovf_bin(float mv, float fx, float rho_t, float cap)
 
 // Adds the overlaps of rectangle r (in bin areas, scaled by sc) to plane
 // `plane` of the LDS window (x0, y0, nx, ny) when r lies inside it, else
-// straight into `acc`.  The window is the dynamic LDS block, declared here and
+// straight into `acc`.  Every path adds dens_fix(ox * (oy * sc)) per bin, so
+// the maps depend neither on the order of the cells nor on which window (if
+// any) a cell is scattered through.  The window is the dynamic LDS block, declared here and
 // not passed as a pointer so the adds compile to LDS atomics.  Bins of
 // [i0, i1] x [j0, j1] never have a negative overlap, so no test is needed per
 // bin; the y overlaps are shared by every row of the rectangle.
@@ -540,7 +542,9 @@ __device__ inline void scatter_rect(const Rect &r, float sc, int plane,
     for (int i = r.i0; i <= r.i1; ++i) {
         const float ox = ovl(r.tx0, r.tx1, i);
         for (int j = r.j0; j <= r.j1; ++j) {
-            const unsigned long long v = dens_fix(ox * ovl(r.ty0, r.ty1, j) * sc);
+            // the same product as the window path's, so that a bin's value does not
+            // depend on which path a rectangle takes
+            const unsigned long long v = dens_fix(ox * (ovl(r.ty0, r.ty1, j) * sc));
             const unsigned li = (unsigned)(i - x0), lj = (unsigned)(j - y0);
             if (lds && li < (unsigned)nx && lj < (unsigned)ny)
                 atomicAdd(&dens_win[plane * wcap + li * ny + lj], v);
