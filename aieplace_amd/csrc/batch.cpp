@@ -169,7 +169,7 @@ struct ep_batch {
     // One gradient evaluation as a captured fork: the density scatter and the
     // transform products on the side stream beside the WA gradient (the
     // captured stream of the longer branch decides how soon it starts).
-    hipError_t eval_forked(const BDev &d, int at, bool counted, hipStream_t s)
+    hipError_t eval_forked(const BDev &d, int at, bool counted, hipStream_t s, int ctl_mode)
     {
         const int v = dens_variant;
         bl_sort(d, at, counted, bin_blocks, sort_blocks, kDensCpt[v], s);
@@ -182,20 +182,21 @@ struct ep_batch {
         bl_wa(d, at, wa_blocks, s);
         if ((e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
         bl_cell_grad(d, at, cell_blocks, s);
-        bl_reduce(d, s);
+        bl_reduce_control(d, ctl_mode, s);
         return hipSuccess;
     }
 
     // One gradient evaluation of every unfinished instance (`counted`: the
-    // preceding trial move counted the sort tiles).
-    void eval(const BDev &d, int at, bool counted, hipStream_t s)
+    // preceding trial move counted the sort tiles), then the reductions and
+    // the control step `ctl_mode` on them.
+    void eval(const BDev &d, int at, bool counted, hipStream_t s, int ctl_mode)
     {
         bl_maps(d, at, counted, bin_blocks, sort_blocks, dens_blocks[dens_variant],
                 kDensCpt[dens_variant], f16(), s);
         bl_wa(d, at, wa_blocks, s);
         launch_field_batch(bf, prm.precision, s);
         bl_cell_grad(d, at, cell_blocks, s);
-        bl_reduce(d, s);
+        bl_reduce_control(d, ctl_mode, s);
     }
 };
 
@@ -625,21 +626,17 @@ extern "C" ep_status ep_batch_init(ep_batch *b)
     bl_move(d, kMoveInitU, b->sort_blocks, s);
     bl_maps(d, kAtCur, false, b->bin_blocks, b->sort_blocks, b->dens_blocks[b->dens_variant],
             kDensCpt[b->dens_variant], b->f16(), s);
-    bl_reduce(d, s);
-    bl_control(d, kCtlGamma, s);
+    bl_reduce_control(d, kCtlGamma, s);
     // gradient pieces at v_0 with lambda = 0: |gW|_1 and |F|_1
     bl_wa(d, kAtCur, b->wa_blocks, s);
     launch_field_batch(b->bf, b->prm.precision, s);
     bl_cell_grad(d, kAtCur, b->cell_blocks, s);
-    bl_reduce(d, s);
-    bl_control(d, kCtlLambda, s);
+    bl_reduce_control(d, kCtlLambda, s);
     // preconditioned gradient with lambda_0, then the probe displacement
     bl_cell_grad(d, kAtCur, b->cell_blocks, s);
-    bl_reduce(d, s);
-    bl_control(d, kCtlProbe, s);
+    bl_reduce_control(d, kCtlProbe, s);
     bl_move(d, kMoveProbe, b->sort_blocks, s);
-    b->eval(d, kAtTrial, false, s);
-    bl_control(d, kCtlAlpha, s);
+    b->eval(d, kAtTrial, false, s, kCtlAlpha);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(EP_EDEVICE, "ep_batch_init: launch failed: %s", hipGetErrorString(e));
     ep_status st = pull_state(b);
@@ -661,9 +658,9 @@ static ep_status build_graph(ep_batch *b)
     hipError_t e = hipSuccess;
     for (int t = 0; t < kTrialsPerLaunch && e == hipSuccess; ++t) {
         bl_move(d, kMoveTrial, b->sort_blocks, b->stream);
-        if (b->active_cells() >= fork_min_cells(b->prm)) e = b->eval_forked(d, kAtTrial, true, b->stream);
-        else b->eval(d, kAtTrial, true, b->stream);
-        bl_control(d, kCtlIterate, b->stream);
+        if (b->active_cells() >= fork_min_cells(b->prm))
+            e = b->eval_forked(d, kAtTrial, true, b->stream, kCtlIterate);
+        else b->eval(d, kAtTrial, true, b->stream, kCtlIterate);
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {  // leave capture mode before reporting

@@ -122,8 +122,9 @@ void bl_maps(const BDev &d, int at, bool counted, int bin_blocks, int sort_block
              int dens_cpt, bool map_range, hipStream_t s);
 void bl_wa(const BDev &d, int at, int wa_blocks, hipStream_t s);
 void bl_cell_grad(const BDev &d, int at, int cell_blocks, hipStream_t s);
-void bl_reduce(const BDev &d, hipStream_t s);
-void bl_control(const BDev &d, int mode, hipStream_t s);
+// The per-instance FP64 reductions of an evaluation, then the control step
+// `mode` (BCtlMode) of every unfinished instance on them.
+void bl_reduce_control(const BDev &d, int mode, hipStream_t s);
 bool bl_density_cpt_supported(int cpt);
 
 // ---- batched field solve (k_field.hip)

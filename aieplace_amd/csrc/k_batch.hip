@@ -368,11 +368,10 @@ __global__ __launch_bounds__(kBlock) void kb_cell_grad(BDev d, int at)
 
 // ------------------------------------------------------------------ reduction + control
 
-// One block per instance: fixed-order FP64 sums of the instance's partials.
-__global__ __launch_bounds__(kBlock) void kb_reduce(BDev d)
+// Block i: fixed-order FP64 sums of instance i's partials into d.res[i]
+// (thread 0 writes them).
+__device__ inline void reduce_one(const BDev &d, int i)
 {
-    const int i = blockIdx.x;
-    if (d.st[i].done) return;
     // hpwl, wa, ovf, dv2, dg2, |gw|, |f|, g2, nonfinite ; max |g|
     double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = d.pre_wa[i] + threadIdx.x; b < d.pre_wa[i + 1]; b += kBlock) {
@@ -427,14 +426,12 @@ __device__ inline void begin_iteration(BState &S)
     S.f_lambda = (float)S.lambda;
 }
 
-// One thread per instance.  Initialisation steps and the per-trial decision of
-// ep_step: accept when alpha_hat >= 0.95 alpha_used (or after max_backtrack
+// Instance i's initialisation steps and per-trial decision of ep_step (one
+// thread): accept when alpha_hat >= 0.95 alpha_used (or after max_backtrack
 // trials), then lambda *= mu(dHPWL), gamma(tau), stop at overflow / max_iters.
-__global__ void kb_control(BDev d, int mode)
+__device__ inline void control_one(const BDev &d, int mode, int i)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.n_inst) return;
     BState S = d.st[i];
     if (S.done) return;
     const BRes R = d.res[i];
@@ -511,7 +508,15 @@ __global__ void kb_control(BDev d, int mode)
     d.st[i] = S;
 }
 
-int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+// One block per instance: block i sums instance i's partials, and its thread 0
+// then takes the instance's control step `mode` on them.
+__global__ __launch_bounds__(kBlock) void kb_reduce_control(BDev d, int mode)
+{
+    const int i = blockIdx.x;
+    if (d.st[i].done) return;
+    reduce_one(d, i);
+    if (threadIdx.x == 0) control_one(d, mode, i);
+}
 
 }  // namespace
 
@@ -562,14 +567,9 @@ void bl_cell_grad(const BDev &d, int at, int cell_blocks, hipStream_t s)
     hipLaunchKernelGGL(kb_cell_grad, dim3(cell_blocks), dim3(kBlock), 0, s, d, at);
 }
 
-void bl_reduce(const BDev &d, hipStream_t s)
+void bl_reduce_control(const BDev &d, int mode, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_reduce, dim3(d.n_inst), dim3(kBlock), 0, s, d);
-}
-
-void bl_control(const BDev &d, int mode, hipStream_t s)
-{
-    hipLaunchKernelGGL(kb_control, dim3(ceil_div(d.n_inst, 64)), dim3(64), 0, s, d, mode);
+    hipLaunchKernelGGL(kb_reduce_control, dim3(d.n_inst), dim3(kBlock), 0, s, d, mode);
 }
 
 }  // namespace ep
