@@ -142,6 +142,38 @@ def test_batch_run_matches_single():
         assert np.array_equal(xs, xb[i][0]) and np.array_equal(ys, xb[i][1]), i
 
 
+def test_batch_partial_runs_continue_exactly():
+    """ep_batch_run with max_launches stops after exactly that many graph
+    replays, although the host keeps the next replay queued while it reads a
+    replay's state back; a later call continues from there.  A run split into
+    pieces therefore ends where one uninterrupted run does, bit for bit."""
+    counts, bins = [5_000, 8_000, 11_000], [128, 128, 128]
+    say(f"case batch partial runs counts={counts}")
+    nls, p = small_batch(counts, bins)
+    with BatchPlacer(nls, bins, p) as bp:
+        whole = bp.run()
+        ref = bp.results()
+        ref_pos = [bp.positions(i) for i in range(len(counts))]
+    with BatchPlacer(nls, bins, p) as bp:
+        done = 0
+        for k in (1, 2, 5):
+            info = bp.run(max_launches=k)
+            say(f"  run(max_launches={k}): {info['launches']} replays, "
+                f"{info['instance_iterations']} instance-iterations so far")
+            assert info["launches"] == k and info["finished"] == 0
+            done += k
+        rest = bp.run()
+        res = bp.results()
+        say(f"  rest: {rest['launches']} replays; whole run: {whole['launches']}")
+        assert rest["finished"] == len(counts)
+        assert done + rest["launches"] == whole["launches"]
+        for i in range(len(counts)):
+            assert res["iterations"][i] == ref["iterations"][i] and res["evals"][i] == ref["evals"][i], i
+            assert res["hpwl"][i] == ref["hpwl"][i] and res["overflow"][i] == ref["overflow"][i], i
+            x, y = bp.positions(i)
+            assert np.array_equal(x, ref_pos[i][0]) and np.array_equal(y, ref_pos[i][1]), i
+
+
 def test_batch_reset_restarts_from_initial_placement():
     counts, bins = [5_000, 7_000], [128, 128]
     say(f"case batch reset counts={counts}")
