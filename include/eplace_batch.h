@@ -1,12 +1,13 @@
 /*
  * eplace_batch.h -- many independent placement instances advanced together.
  *
- * All instances live in one device context.  Each Nesterov trial of every
- * unfinished instance runs as one replay of a captured hipGraph.  Every kernel
+ * All instances live in one device context.  Four Nesterov trials of every
+ * unfinished instance run as one replay of a captured hipGraph.  Every kernel
  * node of that graph covers all instances at once, with blocks mapped to
- * (instance, tile) work items, and a device-side control kernel applies each
- * instance's backtracking decision and lambda/gamma schedules without a host
- * round trip.  The host polls completion every few replays.  Tile shapes (density
+ * (instance, tile) work items, and a device-side kernel reduces each
+ * instance's sums and applies its backtracking decision and lambda/gamma
+ * schedules without a host round trip.  The host reads the instances' state
+ * back after every replay, with the next replay already queued.  Tile shapes (density
  * cells per block, the transform products' K step) are picked by a timing
  * autotuner at creation.
  * No counterpart in the reference (single instance, AIEplace.cpp:44-76).
@@ -48,7 +49,8 @@ ep_status ep_batch_reset(ep_batch *b);
  * batched); ep_batch_run calls it when needed. */
 ep_status ep_batch_init(ep_batch *b);
 /* Runs until every instance stops (overflow or max_iters) or max_launches graph
- * replays (each replay advances every unfinished instance by up to 4 trials). */
+ * replays (each replay advances every unfinished instance by up to 4 trials).
+ * A later call continues where this one stopped. */
 ep_status ep_batch_run(ep_batch *b, int32_t max_launches, ep_batch_run_info *info);
 /* Per-instance results (arrays of n; any may be NULL). */
 ep_status ep_batch_results(ep_batch *b, int32_t *iterations, int64_t *evals, double *hpwl,
