@@ -1217,8 +1217,8 @@ extern "C" ep_status ep_step(ep_ctx *c, ep_step_info *info)
 // of one while the next one runs, so the GPU does not idle for the host's
 // round trip.  After the stop the remaining trials return at once (halt flag).
 
-constexpr int kCtlTrialsPerGraph = 4;
-constexpr int kCtlReplays = 2;
+constexpr int kCtlTrialsPerGraph = 8;  // one replay per read-back (a graph boundary costs ~3.5 us)
+constexpr int kCtlReplays = 1;
 
 static CtlParams ctl_params(const ep_ctx *c)
 {
