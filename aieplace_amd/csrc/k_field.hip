@@ -285,10 +285,14 @@ __device__ inline void sym_f32_tile(const GemmArgs &a, int z, int h0, int n0)
     sym_store<INV, EPI>(a, z, h0, n0, acc, 1.0f, &St[0].a[0][0][0]);
 }
 
+// The single-instance products run beside the WA branch's gathers, whose waves
+// share the SIMDs; their waves take issue priority over them (the gathers have
+// slack until the join).
 template <bool INV, int EPI, int SK>
 __global__ __launch_bounds__(256) void k_sym_f32(GemmArgs a)
 {
     if (a.skip && *a.skip) return;
+    __builtin_amdgcn_s_setprio(3);
     sym_f32_tile<INV, EPI, SK, true>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
@@ -488,6 +492,7 @@ template <bool INV, int EPI, bool F16, int SK>
 __global__ __launch_bounds__(256) void k_sym_split(GemmArgs a)
 {
     if (a.skip && *a.skip) return;
+    __builtin_amdgcn_s_setprio(3);  // as k_sym_f32
     sym_split_tile<INV, EPI, F16, SK, true>(a, blockIdx.z, blockIdx.y * SBM, blockIdx.x * SBN);
 }
 
@@ -500,6 +505,7 @@ __global__ __launch_bounds__(256) void k_sym_f32_b(const GemmArgs *__restrict__ 
     const int4 t = tiles[blockIdx.x];
     const GemmArgs &a = args[t.x];
     if (a.skip && *a.skip) return;
+    __builtin_amdgcn_s_setprio(3);  // as k_sym_f32 (the batch forks its WA gradient too)
     sym_f32_tile<INV, EPI, SK, false>(a, t.w, t.y, t.z);
 }
 
@@ -510,6 +516,7 @@ __global__ __launch_bounds__(256) void k_sym_split_b(const GemmArgs *__restrict_
     const int4 t = tiles[blockIdx.x];
     const GemmArgs &a = args[t.x];
     if (a.skip && *a.skip) return;
+    __builtin_amdgcn_s_setprio(3);  // as k_sym_f32 (the batch forks its WA gradient too)
     sym_split_tile<INV, EPI, F16, SK, false>(a, t.w, t.y, t.z);
 }
 
