@@ -393,6 +393,8 @@ def main():
             # headline form; the products' roofline is reported either way
             _, roof_t = roofline_of(stats, traffic, bf16, item="field_transforms")
             out["roofline_transforms"] = dict(kernel="field_transforms", form=name, **roof_t)
+            if roof_t["traffic"] is not None:
+                out["roofline_transforms"]["traffic_source"] = f"profiles/traffic_{args.workload}_{tag}.md"
         kern[name] = {q["name"]: dict(avg_us=round(q["avg_us"], 2), launches=q["launches"]) for q in stats}
     out["roofline_flops"] = ("transform products count algorithmic flops: the six dense products of a "
                              "solve, 6 Mx My (Mx + My) = 12 M^3; split-BF16 held to the BF16 peak")
