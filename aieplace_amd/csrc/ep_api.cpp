@@ -1465,7 +1465,11 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
     const double gemm_flops = 6.0 * Mxd * Myd * (Mxd + Myd);
     // reads 6 maps, writes 7; the halves of the six matrices: 1.5 (Mx^2 + My^2) floats
     const double field_bytes = 4.0 * 13 * nb + 4.0 * 1.5 * (Mxd * Mxd + Myd * Myd);
-    Item items[] = {
+    // Instances that fork their captured evaluation sum the pin partials per cell
+    // on the WA branch (k_pin_sum), and the cell gradient reads those sums: both
+    // are timed that way here, as the placement runs them.
+    const bool forked = c->n_place >= fork_min_cells(c->prm);
+    std::vector<Item> items = {
         // positions and sizes read twice, packed cells written once
         {"cell_sort", 3, 2 * 16.0 * np + 16.0 * np + 8.0 * kSortTiles, 0},
         // packed cells read; two 64-bit accumulator maps flushed
@@ -1475,11 +1479,18 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
         {"overflow", 1, 44.0 * nb, 0},
         {"wa_grad", 1, 4.0 * (N + 1) + 12.0 * P + 8.0 * c->n_all + 8.0 * P, 0},
         {"field_transforms", 4, field_bytes, gemm_flops},
-        {"cell_grad", 1, 36.0 * np + 12.0 * pm + 4.0 * c->n_mov + 8.0 * nb, 0},
-        // v, u, g, size read; v, u written (the tile counts stay in LDS)
-        {"nesterov_trial", 1, 40.0 * np, 0},
-        {"finalize", 1, 8.0 * c->part_total, 0},
     };
+    if (forked) {
+        // CSR offsets and pin list read, the pins' partials gathered, one sum
+        // per movable cell written
+        items.push_back({"pin_sum", 1, 4.0 * (c->n_mov + 1) + 12.0 * pm + 8.0 * c->n_mov, 0});
+        items.push_back({"cell_grad", 1, 36.0 * np + 8.0 * c->n_mov + 8.0 * nb, 0});
+    } else {
+        items.push_back({"cell_grad", 1, 36.0 * np + 12.0 * pm + 4.0 * c->n_mov + 8.0 * nb, 0});
+    }
+    // v, u, g, size read; v, u written (the tile counts stay in LDS)
+    items.push_back({"nesterov_trial", 1, 40.0 * np, 0});
+    items.push_back({"finalize", 1, 8.0 * c->part_total, 0});
     int k = 0;
     for (const Item &it : items) {
         EP_HIP(hipEventRecord(e0, c->stream));
@@ -1490,7 +1501,11 @@ extern "C" ep_status ep_profile(ep_ctx *c, int32_t reps, ep_kernel_stat *out, in
             else if (nm == "overflow") launch_overflow(*c, c->stream);
             else if (nm == "wa_grad") launch_wa(*c, c->V[a], c->stream);
             else if (nm == "field_transforms") launch_field(*c, c->prm.precision, c->stream);
-            else if (nm == "cell_grad") launch_cell_grad(*c, c->V[a], nullptr, c->G[a], c->G[b], nullptr, nullptr, c->stream);
+            else if (nm == "pin_sum") launch_pin_sum(*c, c->stream);
+            else if (nm == "cell_grad")
+                launch_cell_grad(*c, same<const float2>(c->V[a]), same<const float2>(nullptr),
+                                 same<const float2>(c->G[a]), same<float2>(c->G[b]), nullptr, nullptr, c->stream,
+                                 forked ? c->gw_cell : nullptr);
             else if (nm == "nesterov_trial") launch_trial(*c, c->V[a], c->U[a], c->G[a], c->V[b], c->U[b], true, c->stream);
             else launch_finalize(*c, c->stream, true);
         }

@@ -46,7 +46,8 @@ PREC_NAME = {0: "fp32", 1: "bf16x3", 2: "fp16x3"}
 FAMILIES = [
     ("k_sort_", "cell_sort"), ("k_density", "density_scatter"), ("k_overflow", "overflow"),
     ("k_map_stats", "overflow"), ("k_wa", "wa_grad"), ("k_sym_", "field_transforms"),
-    ("k_clear_max", "field_transforms"), ("k_cell_grad", "cell_grad"), ("k_trial", "nesterov_trial"),
+    ("k_clear_max", "field_transforms"), ("k_pin_sum", "pin_sum"), ("k_cell_grad", "cell_grad"),
+    ("k_trial", "nesterov_trial"),
     ("k_finalize", "finalize"),
 ]
 
