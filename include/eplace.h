@@ -189,7 +189,11 @@ ep_status ep_set_precision(ep_ctx *ctx, int32_t precision);
 /* ---- measurement (replaces the gettimeofday timers of AIEplace.cpp:1077-1085) --
  * Times each kernel group of one gradient evaluation at the current state with
  * HIP events on the context's stream, `reps` launches each, and reports the
- * algorithmic HBM bytes and flops of one launch. */
+ * algorithmic HBM bytes and flops of one launch.  Groups: cell_sort,
+ * density_scatter, overflow, wa_grad, field_transforms, [pin_sum,] cell_grad,
+ * nesterov_trial, finalize, then the scatter at other cells per thread; contexts
+ * that fork their captured evaluation (fork_min_cells) report pin_sum and a
+ * cell_grad that reads its sums, as the graph runs them. */
 typedef struct ep_kernel_stat {
     char name[32];
     int32_t launches;   /* launches of this group per gradient evaluation        */
