@@ -378,16 +378,16 @@ def test_placement_is_deterministic(precision):
 
 @pytest.mark.parametrize("precision,learned_step,fork", [(api.EP_PREC_FP32, 0, 0), (api.EP_PREC_BF16X3, 0, 0),
                                                          (api.EP_PREC_BF16X3, 1, 0), (api.EP_PREC_FP32, 0, 1),
-                                                         (api.EP_PREC_BF16X3, 1, 1)])
+                                                         (api.EP_PREC_BF16X3, 1, 1), (api.EP_PREC_FP16X3, 0, 1)])
 def test_device_control_matches_host_control(precision, learned_step, fork):
     """ep_run with graphs decides every trial on the device (the step-control
     kernel replays ep_step's FP64 decisions); without graphs ep_run loops over
     the host's ep_step.  Both take the same decisions, so the runs agree
     exactly: iteration and evaluation counts, every logged iteration, the
     step records and the final positions.  fork = 1 lowers the two-branch
-    threshold so the captured evaluation runs the WA gradient and the per-cell
-    pin sums on a second branch (the form every run of >= 400k cells takes);
-    the host loop never forks."""
+    threshold so the captured evaluation runs the density scatter and the field
+    on a second branch beside the WA gradient and the per-cell pin sums (the
+    form every run of >= 400k cells takes); the host loop never forks."""
     from aieplace_amd import learned
     say(f"case device vs host step control precision={precision} learned={learned_step} fork={fork}")
     nl, p = make("stdcell", 20_000, 128, precision=precision, learned_step=learned_step,
