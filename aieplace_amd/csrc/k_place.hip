@@ -453,9 +453,18 @@ struct StepCtlArgs {
     const float *mlp_w;     // learned step: the model proposes the next trial's step
 };
 
+// What the decision leaves for the next trial's model proposal, kept by the
+// deciding thread as it writes the same values to the scalars (so that it
+// need not read them back): the model flag, the halt flag and the features.
+struct NextTrial {
+    int use_model;
+    int halt;
+    float *feat;  // 16 floats, written when the decision starts an iteration
+};
+
 __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevCtl &S0, const DevResult &R,
                                    DevScalars *__restrict__ sc, const CtlParams &p,
-                                   ep_step_info *__restrict__ log, float *__restrict__ rec);
+                                   ep_step_info *__restrict__ log, float *__restrict__ rec, NextTrial &nt);
 
 __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     const double *__restrict__ part, int wa_off, int wa_blocks, int ovf_off, int ovf_blocks,
@@ -470,10 +479,12 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     // the last block does not wait for them (the previous launch wrote them).
     __shared__ DevCtl ctl_s;
     __shared__ float alpha_s;
+    __shared__ int use_model_s;
     uint32_t ctl_w = 0;
     const bool ctl_load = ctl.ctl && threadIdx.x < kCtlWords;
     if (ctl_load) ctl_w = reinterpret_cast<const uint32_t *>(ctl.ctl)[threadIdx.x];
     const float alpha_used = sc->alpha;
+    const int use_model0 = sc->use_model;
     // The step model's weights likewise (learned runs): registers now, LDS
     // after the partial sums, read by the proposal at the end.
     __shared__ float mlp_s[EP_MLP_NWEIGHTS];
@@ -508,7 +519,10 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
     __shared__ double out[10];
     __shared__ int last;
     if (ctl_load) reinterpret_cast<uint32_t *>(&ctl_s)[threadIdx.x] = ctl_w;
-    if (threadIdx.x == 0) alpha_s = alpha_used;
+    if (threadIdx.x == 0) {
+        alpha_s = alpha_used;
+        use_model_s = use_model0;
+    }
     if (ctl.mlp_w)
 #pragma unroll
         for (int k = 0; k < kMlpWordsPerThread; ++k) {
@@ -558,10 +572,15 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_finalize(
         R.gmax = t[9];
         R.alpha_used = alpha_s;
         *res = R;
-        if (ctl.ctl) step_decide(ctl.ctl, ctl_s, R, sc, ctl.p, ctl.log, ctl.rec);  // device-controlled run
-        propose = ctl.mlp_w && sc->use_model && !sc->halt;
-        if (propose)
-            for (int k = 0; k < 16; ++k) feat[k] = sc->feat[k];
+        if (ctl.ctl) {  // device-controlled run: the decision also fills feat
+            NextTrial nt{use_model_s, 0, feat};
+            step_decide(ctl.ctl, ctl_s, R, sc, ctl.p, ctl.log, ctl.rec, nt);
+            propose = ctl.mlp_w && nt.use_model && !nt.halt;
+        } else {
+            propose = ctl.mlp_w && sc->use_model && !sc->halt;
+            if (propose)
+                for (int k = 0; k < 16; ++k) feat[k] = sc->feat[k];
+        }
     }
     // the model's step for the next trial, while this block is still resident
     if (ctl.mlp_w) {
@@ -690,14 +709,17 @@ __device__ inline void ctl_features(const DevCtl &S, double bw, double isn, floa
     f[15] = (float)(S.iteration / 1000.0);
 }
 
-// Nesterov coefficient, features and the scalars of the iteration's first trial.
-__device__ inline void ctl_begin(DevCtl &S, DevScalars *sc, const CtlParams &p)
+// Nesterov coefficient, features and the scalars of the iteration's first trial
+// (nt, when given, gets the model flag and the features too).
+__device__ inline void ctl_begin(DevCtl &S, DevScalars *sc, const CtlParams &p, NextTrial *nt = nullptr)
 {
 #pragma clang fp contract(off)
     S.ak1 = 0.5 * (1.0 + sqrt(4.0 * S.ak * S.ak + 1.0));
     S.coef = (S.ak - 1.0) / S.ak1;
     ctl_features(S, p.bw, p.inv_sqrt_np, S.feat);
     for (int k = 0; k < 16; ++k) sc->feat[k] = S.feat[k];
+    if (nt)
+        for (int k = 0; k < 16; ++k) nt->feat[k] = S.feat[k];
     S.trials = 0;
     S.t = 0;
     S.learned_ok = 0;
@@ -711,23 +733,28 @@ __device__ inline void ctl_begin(DevCtl &S, DevScalars *sc, const CtlParams &p)
     sc->gamma_inv = (float)(1.0 / S.gamma);
     sc->lambda = (float)S.lambda;
     sc->use_model = propose ? 1 : 0;
+    if (nt) nt->use_model = propose ? 1 : 0;
 }
 
 // The decision after a trial whose reductions are R (one thread); S0 is the
 // control state *ctl as the trial left it, the result goes back to *ctl.
 __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevCtl &S0, const DevResult &R,
                                    DevScalars *__restrict__ sc, const CtlParams &p,
-                                   ep_step_info *__restrict__ log, float *__restrict__ rec)
+                                   ep_step_info *__restrict__ log, float *__restrict__ rec, NextTrial &nt)
 {
 #pragma clang fp contract(off)
     DevCtl S = S0;
-    if (S.done) return;
+    if (S.done) {  // (the halt flag was set with it)
+        nt.halt = 1;
+        return;
+    }
     S.evals++;
     S.trials++;
     if (R.nonfinite) {
         S.error = 1;
         S.done = 1;
         sc->halt = 1;
+        nt.halt = 1;
         *ctl = S;
         return;
     }
@@ -747,6 +774,7 @@ __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevCtl &S0, c
             S.phase = 0;
             S.t = 0;
             sc->use_model = 0;
+            nt.use_model = 0;
             sc->alpha = (float)S.alpha;
         }
     } else if (ahat >= 0.95 * used || S.t == p.max_backtrack - 1) {
@@ -806,8 +834,9 @@ __device__ inline void step_decide(DevCtl *__restrict__ ctl, const DevCtl &S0, c
         if (tau <= p.stop_overflow || S.iteration >= p.max_iters) {
             S.done = 1;
             sc->halt = 1;
+            nt.halt = 1;
         } else {
-            ctl_begin(S, sc, p);
+            ctl_begin(S, sc, p, &nt);
         }
     }
     *ctl = S;
