@@ -1274,12 +1274,14 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
         // the model's proposal for this trial was made by the previous decision
         // (k_finalize) or by k_step_begin
         launch_trial(x, vc, uc, gc, vn, un, true, s);
-        launch_sort(x, vt, true, s);
         if (x.n_place >= fork_min_cells(x.prm)) {
-            // fork: the density scatter and the transform products on the second
-            // stream, the WA gradient and the per-cell pin sums beside them
+            // fork right after the trial: the tile sort, the density scatter and
+            // the transform products on the second stream (the critical chain;
+            // the products take issue priority), the WA gradient and the
+            // per-cell pin sums beside them
             EP_CAP(hipEventRecord(x.fork, s), s);
             EP_CAP(hipStreamWaitEvent(x.side, x.fork, 0), s);
+            launch_sort(x, vt, true, x.side);
             launch_density(x, x.side);
             launch_overflow(x, x.side);
             launch_field(x, x.prm.precision, x.side);
@@ -1289,6 +1291,7 @@ static ep_status build_ctl_graph(ep_ctx *c, const CtlParams &p)
             EP_CAP(hipStreamWaitEvent(s, x.join, 0), s);
             launch_cell_grad(x, vt, none, gc, gn, nullptr, nullptr, s, x.gw_cell);
         } else {
+            launch_sort(x, vt, true, s);
             launch_density(x, s);
             launch_overflow(x, s);
             launch_wa(x, vt, s);
