@@ -166,16 +166,17 @@ struct ep_batch {
         return n_cells;
     }
 
-    // One gradient evaluation as a captured fork: the density scatter and the
-    // transform products on the side stream beside the WA gradient (the
-    // captured stream of the longer branch decides how soon it starts).
+    // One gradient evaluation as a captured fork: the tile sort, the density
+    // scatter and the transform products on the side stream beside the WA
+    // gradient (the captured stream of the longer branch decides how soon it
+    // starts), as in the single-instance graph.
     hipError_t eval_forked(const BDev &d, int at, bool counted, hipStream_t s, int ctl_mode)
     {
         const int v = dens_variant;
-        bl_sort(d, at, counted, bin_blocks, sort_blocks, kDensCpt[v], s);
         hipError_t e = hipEventRecord(fork, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
         if (e != hipSuccess) return e;
+        bl_sort(d, at, counted, bin_blocks, sort_blocks, kDensCpt[v], side);
         bl_dens(d, bin_blocks, dens_blocks[v], kDensCpt[v], f16(), side);
         launch_field_batch(bf, prm.precision, side);
         if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
