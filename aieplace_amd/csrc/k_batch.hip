@@ -75,24 +75,47 @@ __global__ __launch_bounds__(kBlock) void kb_move(BDev d, int mode)
     const int a = S->cur;
     const GeomArgs g = inst_geom(I);
     const int base = (blockIdx.x - d.pre_sort[i]) * kBlock * kSortCellsPerThread;
+    if (mode == kMoveTrial) {
+        // All of the thread's cells are loaded before the first is moved, so
+        // the loads overlap; a cell past n_place loads the instance's last
+        // cell (value unused).
+        const float alpha = S->f_alpha, coef = S->f_coef;
+        const float2 *V = pick<const float2 *>(d.V[0], d.V[1], a);
+        const float2 *U = pick<const float2 *>(d.U[0], d.U[1], a);
+        const float2 *G = pick<const float2 *>(d.G[0], d.G[1], a);
+        float2 s[kSortCellsPerThread], vv[kSortCellsPerThread], uu[kSortCellsPerThread],
+            gg[kSortCellsPerThread];
 #pragma unroll
-    for (int k = 0; k < kSortCellsPerThread; ++k) {
+        for (int k = 0; k < kSortCellsPerThread; ++k) {
+            const int c = I.cell_off + min(base + k * kBlock + (int)threadIdx.x, I.n_place - 1);
+            s[k] = d.wh[c];
+            vv[k] = V[c];
+            uu[k] = U[c];
+            gg[k] = G[c];
+        }
+#pragma unroll
+        for (int k = 0; k < kSortCellsPerThread; ++k) {
+            const int lc = base + k * kBlock + threadIdx.x;
+            const float2 un =
+                clamp_cell(make_float2(vv[k].x - alpha * gg[k].x, vv[k].y - alpha * gg[k].y), s[k], g);
+            const float2 vn = clamp_cell(
+                make_float2(un.x + coef * (un.x - uu[k].x), un.y + coef * (un.y - uu[k].y)), s[k], g);
+            if (lc < I.n_place) {
+                const int c = I.cell_off + lc;
+                pick(d.U[0], d.U[1], a ^ 1)[c] = un;
+                pick(d.V[0], d.V[1], a ^ 1)[c] = vn;
+                atomicAdd(&hist[cell_key(vn, s[k], g, I.M, I.M)], 1);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread && mode != kMoveTrial; ++k) {
         const int lc = base + k * kBlock + threadIdx.x;
         if (lc >= I.n_place) continue;
         const int c = I.cell_off + lc;
         const float2 s = d.wh[c];
         const float2 vv = pick<const float2 *>(d.V[0], d.V[1], a)[c];
-        if (mode == kMoveTrial) {
-            const float alpha = S->f_alpha, coef = S->f_coef;
-            const float2 uu = pick<const float2 *>(d.U[0], d.U[1], a)[c];
-            const float2 gg = pick<const float2 *>(d.G[0], d.G[1], a)[c];
-            const float2 un = clamp_cell(make_float2(vv.x - alpha * gg.x, vv.y - alpha * gg.y), s, g);
-            const float2 vn =
-                clamp_cell(make_float2(un.x + coef * (un.x - uu.x), un.y + coef * (un.y - uu.y)), s, g);
-            pick(d.U[0], d.U[1], a ^ 1)[c] = un;
-            pick(d.V[0], d.V[1], a ^ 1)[c] = vn;
-            atomicAdd(&hist[cell_key(vn, s, g, I.M, I.M)], 1);
-        } else if (mode == kMoveProbe) {
+        if (mode == kMoveProbe) {
             const float eps = S->f_eps;
             const float2 gg = pick<const float2 *>(d.G[0], d.G[1], a)[c];
             pick(d.V[0], d.V[1], a ^ 1)[c] =
@@ -221,21 +244,31 @@ __global__ __launch_bounds__(kBlock) void kb_overflow(BDev d)
     const BInst &I = d.inst[i];
     const int nb = I.M * I.M;
     const int base = (blockIdx.x - d.pre_bin[i]) * kBinsPerBlock;
+    constexpr int K = kBinsPerBlock / kBlock;
     double acc[4] = {0.0, 0.0, -1e300, -1e300};
+    // All K bins' loads are issued before the first conversion; a bin past
+    // the instance's map loads its last bin instead (in bounds, value unused).
+    float fxs[K];
+    unsigned long long ams[K], afs[K];
 #pragma unroll
-    for (int k = 0; k < kBinsPerBlock / kBlock; ++k) {
+    for (int k = 0; k < K; ++k) {
+        const size_t o = (size_t)I.map_off + min(base + k * kBlock + (int)threadIdx.x, nb - 1);
+        fxs[k] = d.map_fix[o];
+        ams[k] = d.acc_mov[o];
+        afs[k] = d.acc_field[o];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
         const int b = base + k * kBlock + threadIdx.x;
+        const float mv = mov_bin(ams[k], I.dens_unit);
+        const float fd = field_bin(fxs[k], afs[k], I.dens_unit, I.rho_t);
+        const float v = ovf_bin(mv, fxs[k], I.rho_t, I.cap);
         if (b < nb) {
             const size_t o = (size_t)I.map_off + b;
-            const float fx = d.map_fix[o];
-            const unsigned long long am = d.acc_mov[o], af = d.acc_field[o];
             d.acc_mov[o] = 0ull;  // cleared for the next scatter
             d.acc_field[o] = 0ull;
-            const float mv = mov_bin(am, I.dens_unit);
-            const float fd = field_bin(fx, af, I.dens_unit, I.rho_t);
             d.map_mov[o] = mv;
             d.map_field[o] = fd;
-            const float v = ovf_bin(mv, fx, I.rho_t, I.cap);
             if (v > 0.0f) acc[0] += (double)v;
             const double f = fd;
             acc[1] += f;
