@@ -185,16 +185,23 @@ __global__ __launch_bounds__(kBlock) void kb_sort_scatter(BDev d, int at)
     const int base = (blockIdx.x - d.pre_sort[i]) * kBlock * kSortCellsPerThread;
     int key[kSortCellsPerThread], rank[kSortCellsPerThread];
     float4 cell[kSortCellsPerThread];
+    // All of the thread's cells are loaded before the first tile count; a
+    // cell past n_place loads the instance's last cell (value unused).
+    float2 p[kSortCellsPerThread], s[kSortCellsPerThread];
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k) {
+        const int c = I.cell_off + min(base + k * kBlock + (int)threadIdx.x, I.n_place - 1);
+        p[k] = pos[c];
+        s[k] = d.wh[c];
+    }
 #pragma unroll
     for (int k = 0; k < kSortCellsPerThread; ++k) {
         const int idx = base + k * kBlock + threadIdx.x;
         key[k] = -1;
         if (idx < I.n_place) {
-            const int c = I.cell_off + idx;
-            const float2 p = pos[c], s = d.wh[c];
-            key[k] = cell_key(p, s, g, I.M, I.M);
+            key[k] = cell_key(p[k], s[k], g, I.M, I.M);
             rank[k] = atomicAdd(&hist[key[k]], 1);
-            cell[k] = pack_cell(p, s, idx < I.n_mov);
+            cell[k] = pack_cell(p[k], s[k], idx < I.n_mov);
         }
     }
     __syncthreads();

@@ -100,15 +100,23 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(Pair<const float2> pos_
     const int base = blockIdx.x * kBlock * kSortCellsPerThread;
     int key[kSortCellsPerThread], rank[kSortCellsPerThread];
     float4 cell[kSortCellsPerThread];
+    // All of the thread's cells are loaded before the first tile count, so
+    // the loads overlap; a cell past n_place loads the last cell (unused).
+    float2 p[kSortCellsPerThread], s[kSortCellsPerThread];
+#pragma unroll
+    for (int k = 0; k < kSortCellsPerThread; ++k) {
+        const int idx = min(base + k * kBlock + (int)threadIdx.x, n_place - 1);
+        p[k] = pos[idx];
+        s[k] = wh[idx];
+    }
 #pragma unroll
     for (int k = 0; k < kSortCellsPerThread; ++k) {
         const int idx = base + k * kBlock + threadIdx.x;
         key[k] = -1;
         if (idx < n_place) {
-            const float2 p = pos[idx], s = wh[idx];
-            key[k] = cell_key(p, s, g, Mx, My);
+            key[k] = cell_key(p[k], s[k], g, Mx, My);
             rank[k] = atomicAdd(&hist[key[k]], 1);
-            cell[k] = pack_cell(p, s, idx < n_mov);
+            cell[k] = pack_cell(p[k], s[k], idx < n_mov);
         }
     }
     __syncthreads();
