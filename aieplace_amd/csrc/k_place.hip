@@ -386,6 +386,10 @@ __global__ void k_wa_packet_fast64(const float *__restrict__ data, int net_size,
 
 // g = (gW - lambda * F) / max(1, pins + lambda * area); F = r * sum ov * E over
 // the smoothed rectangle.  Also the FP64 partial norms for the step length.
+// kVref / kGref / kGwCell: whether vref_pair, gref_pair and gw_cell are set.
+// As compile-time flags they leave the loads of a cell in one basic block,
+// so they are in flight together.
+template <bool kVref, bool kGref, bool kGwCell>
 __global__ __launch_bounds__(kBlock) void k_cell_grad(
     Pair<const float2> pos_pair, const float2 *__restrict__ wh, int n_place, int n_mov,
     GeomArgs g, int Mx, int My, const int32_t *__restrict__ cell_pin_start,
@@ -406,16 +410,17 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     auto one = [&](int c) {
         const float2 p = pos[c], s = wh[c];
-        const float2 v0 = vref ? vref[c] : p;  // independent loads first
-        const float2 g0 = gref ? gref[c] : make_float2(0.f, 0.f);
-        int k0 = 0, k1 = 0;
-        if (c < n_mov) {
-            k0 = cell_pin_start[c];
-            k1 = cell_pin_start[c + 1];
-        }
+        const float2 v0 = kVref ? vref[c] : p;  // independent loads first
+        const float2 g0 = kGref ? gref[c] : make_float2(0.f, 0.f);
+        // Clamped instead of branching on c < n_mov: cell_pin_start holds
+        // n_mov + 1 entries, so a fixed cell or filler reads k0 == k1 (no pins).
+        const bool mov = c < n_mov;
+        const int k0 = cell_pin_start[min(c, n_mov)], k1 = cell_pin_start[min(c + 1, n_mov)];
+        float2 gwc = make_float2(0.f, 0.f);
+        if (kGwCell) gwc = gw_cell[max(0, min(c, n_mov - 1))];  // holds max(1, n_mov) entries
         const int np = k1 - k0;
         const float2 f = cell_force(p, s, exy, g, Mx, My);
-        const float2 gw = gw_cell ? (c < n_mov ? gw_cell[c] : make_float2(0.f, 0.f))
+        const float2 gw = kGwCell ? (mov ? gwc : make_float2(0.f, 0.f))
                                   : sum_pin_partials(k0, k1, cell_pins, pin_grad);
         const float gwx = gw.x, gwy = gw.y, fx = f.x, fy = f.y;
         const float pre = fmaxf(1.0f, (float)np + lambda * s.x * s.y);
@@ -423,11 +428,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_grad(
         gout[c] = make_float2(gx, gy);
         if (gw_out) gw_out[c] = make_float2(gwx, gwy);
         if (f_out) f_out[c] = make_float2(fx, fy);
-        if (vref) {
+        if (kVref) {
             const double dx = (double)p.x - v0.x, dy = (double)p.y - v0.y;
             acc[0] += dx * dx + dy * dy;
         }
-        if (gref) {
+        if (kGref) {
             const double dx = (double)gx - g0.x, dy = (double)gy - g0.y;
             acc[1] += dx * dx + dy * dy;
         }
@@ -982,10 +987,22 @@ void launch_cell_grad(Ctx &c, Pair<const float2> pos, Pair<const float2> vref,
                       Pair<const float2> gref, Pair<float2> gout, float2 *gw_out, float2 *f_out,
                       hipStream_t s, const float2 *gw_cell)
 {
-    hipLaunchKernelGGL(k_cell_grad, dim3(c.cell_blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place,
-                       c.n_mov, geom_args(c), c.Mx, c.My, c.cell_pin_start, c.cell_pins,
-                       c.pin_grad, c.exy, vref, gref, gout, gw_out, f_out, c.d_scal,
-                       c.part + c.part_cell_off, gw_cell);
+    // Every caller sets both entries of a pair or neither.
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(c.cell_blocks), dim3(kBlock), 0, s, pos, c.wh, c.n_place, c.n_mov,
+                           geom_args(c), c.Mx, c.My, c.cell_pin_start, c.cell_pins, c.pin_grad, c.exy,
+                           vref, gref, gout, gw_out, f_out, c.d_scal, c.part + c.part_cell_off, gw_cell);
+    };
+    switch ((vref.p[0] ? 1 : 0) | (gref.p[0] ? 2 : 0) | (gw_cell ? 4 : 0)) {
+    case 0: go(k_cell_grad<false, false, false>); break;
+    case 1: go(k_cell_grad<true, false, false>); break;
+    case 2: go(k_cell_grad<false, true, false>); break;
+    case 3: go(k_cell_grad<true, true, false>); break;
+    case 4: go(k_cell_grad<false, false, true>); break;
+    case 5: go(k_cell_grad<true, false, true>); break;
+    case 6: go(k_cell_grad<false, true, true>); break;
+    default: go(k_cell_grad<true, true, true>); break;
+    }
 }
 
 void launch_finalize(Ctx &c, hipStream_t s, bool trial)
